@@ -1,0 +1,204 @@
+"""Benchmark: complex tensor contraction on one MI355X.
+
+Default workload (the headline): complex<float>
+A[48,48,48,48] 'abcd' x B[48,48,48,48] 'dbef' -> C[48,48,48,48] 'acef'
+(m = n = k = 2304, col-major, no operand GEMM-contiguous), re/im i.i.d.
+U[-1, 1], through t1m_contract on device-resident tensors with the plan
+(index maps) cached.  One step = one contraction call (C += A.B) on the
+FP32-MFMA path.  The same inputs then run on the BF16x3 path, and both
+paths' relative Frobenius errors are measured on log-uniform-magnitude inputs
+against a complex128 reference.
+
+Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2516.6}   # MI355X dense TFLOP/s
+
+CONFIGS = {
+    "c2": dict(la="abcd", lb="dbef", lc="acef", dtype="complex64",
+               dims={ch: 48 for ch in "abcdef"}, peak="fp32",
+               workload="complex<float> abcd,dbef->acef 48^4 (m=n=k=2304), FP32 MFMA 1M"),
+    "c3": dict(la="asb", lb="btc", lc="astc", dtype="complex128",
+               dims={"a": 256, "s": 4, "b": 256, "t": 4, "c": 256}, peak="fp64",
+               workload="complex<double> asb,btc->astc 256x4x256 (m=n=1024,k=256), FP64 MFMA 1M"),
+}
+
+
+def flops(cfg):
+    m = n = k = 1
+    for ch, d in cfg["dims"].items():
+        if ch in cfg["la"] and ch in cfg["lb"]:
+            k *= d
+        elif ch in cfg["la"]:
+            m *= d
+        else:
+            n *= d
+    return 8 * m * n * k, (m, n, k)
+
+
+def make_inputs(cfg, kind, seed, torch):
+    """Col-major device tensors (strides (1, d0, d0*d1, ...))."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    dt = getattr(torch, cfg["dtype"])
+    rdt = torch.float32 if dt == torch.complex64 else torch.float64
+
+    def one(labels):
+        shape = [cfg["dims"][ch] for ch in labels]
+        rev = list(reversed(shape))
+        if kind == "uniform":
+            re = torch.rand(rev, generator=g, device="cuda", dtype=rdt) * 2 - 1
+            im = torch.rand(rev, generator=g, device="cuda", dtype=rdt) * 2 - 1
+            z = torch.complex(re, im)
+        elif kind == "normal":
+            scale = 1.0 / (cfg["dims"].get("b", 16) ** 0.5)
+            z = torch.complex(torch.randn(rev, generator=g, device="cuda", dtype=rdt),
+                              torch.randn(rev, generator=g, device="cuda", dtype=rdt)) * scale
+        else:  # log-uniform magnitude over [1e-3, 1e3], uniform phase
+            r = 10.0 ** (torch.rand(rev, generator=g, device="cuda", dtype=rdt) * 6 - 3)
+            th = torch.rand(rev, generator=g, device="cuda", dtype=rdt) * (2 * torch.pi)
+            z = torch.polar(r, th)
+        z = z.to(dt).contiguous()
+        return z.permute(*reversed(range(len(shape))))
+
+    return one(cfg["la"]), one(cfg["lb"]), one(cfg["lc"]) * 0
+
+
+def time_path(t1m, torch, a, b, c, cfg, precision, steps, warmup):
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision,
+                     synchronize=False)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s, e in evs:
+        s.record(stream)
+        t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision,
+                     synchronize=False)
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [s.elapsed_time(e) for s, e in evs]
+    return wall, kern_ms
+
+
+def rel_errors(t1m, torch, cfg, seed):
+    a, b, _ = make_inputs(cfg, "logmag", seed, torch)
+    ref = torch.einsum(f"{cfg['la']},{cfg['lb']}->{cfg['lc']}",
+                       a.to(torch.complex128), b.to(torch.complex128))
+    out = {}
+    precs = ["native", "bf16x3"] if cfg["dtype"] == "complex64" else ["native"]
+    for p in precs:
+        c = torch.zeros_like(ref, dtype=a.dtype)
+        t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=p)
+        out[p] = (torch.linalg.vector_norm(c.to(torch.complex128) - ref) /
+                  torch.linalg.vector_norm(ref)).item()
+    return out
+
+
+def cpu_baseline(cfg, seconds_hint=15.0):
+    """The oracle's five-loop 1M restatement (single thread) on a bounded
+    slice of the same workload: B's and C's last free mode cut to `cut`
+    values, which scales the flops linearly (8 m n' k)."""
+    import numpy as np
+    from oracle import oracle as orc
+    rng = np.random.default_rng(7)
+    dt = np.complex64 if cfg["dtype"] == "complex64" else np.complex128
+    lastb = [ch for ch in cfg["lc"] if ch not in cfg["la"]][-1]
+    dims = dict(cfg["dims"])
+
+    def arr(labels, d):
+        shape = [d[ch] for ch in labels]
+        x = rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+        return np.asfortranarray(x.astype(dt))
+
+    full, _ = flops(cfg)
+    cut = 1
+    while True:
+        d = dict(dims)
+        d[lastb] = cut
+        A, B, C = arr(cfg["la"], d), arr(cfg["lb"], d), arr(cfg["lc"], d) * 0
+        t0 = time.perf_counter()
+        orc.contract(A, cfg["la"], B, cfg["lb"], C, cfg["lc"])
+        dt_s = time.perf_counter() - t0
+        fl = full * cut / dims[lastb]
+        if dt_s * 2 > seconds_hint or cut * 2 > dims[lastb]:
+            break
+        cut *= 2
+    return {"value": round(fl / dt_s / 1e12, 6), "unit": "TFLOP/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"oracle five-loop 1M contraction (BLIS-sized blocks, plain C "
+                       f"micro-kernel), {cfg['la']},{cfg['lb']}->{cfg['lc']} with "
+                       f"'{lastb}' cut to {cut} of {dims[lastb]} ({fl:.3e} flop, "
+                       f"{dt_s:.2f} s, 1 thread)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-errors", action="store_true", help="skip the error report")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
+              "multi-GPU sharding is design-only.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import t1m_amd as t1m
+
+    cfg = CONFIGS[args.config]
+    fl, (m, n, k) = flops(cfg)
+    a, b, c = make_inputs(cfg, "uniform" if cfg["dtype"] == "complex64" else "normal", 2, torch)
+
+    wall, kern = time_path(t1m, torch, a, b, c, cfg, "native", args.steps, args.warmup)
+    ms_step = wall / args.steps * 1e3
+    value = fl * args.steps / wall / 1e12
+    kern_avg = sum(kern) / len(kern)
+    achieved = fl / (kern_avg * 1e-3) / 1e12
+    peak = PEAK[cfg["peak"]]
+    line = {
+        "metric": "complex contraction TFLOP/s, % MFMA peak; BF16x3 vs FP32 at rel err <=1e-5",
+        "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32" if cfg["dtype"] == "complex64" else "fp64",
+        "data": "synthetic (seeded, re/im i.i.d. U[-1,1]; device-resident)",
+        "config": {"workload": cfg["workload"], "labels": f"{cfg['la']},{cfg['lb']}->{cfg['lc']}",
+                   "m": m, "n": n, "k": k, "flop_per_step": fl, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel_ms": round(kern_avg, 4)},
+    }
+    if cfg["dtype"] == "complex64":
+        c2 = torch.zeros_like(c)
+        wall_b, kern_b = time_path(t1m, torch, a, b, c2, cfg, "bf16x3", args.steps, args.warmup)
+        kb = sum(kern_b) / len(kern_b)
+        line["bf16x3"] = {"value": round(fl * args.steps / wall_b / 1e12, 3), "unit": "TFLOP/s",
+                          "kernel_ms": round(kb, 4),
+                          "speedup_vs_fp32": round(kern_avg / kb, 3),
+                          "frac_of_fp32_peak": round(fl / (kb * 1e-3) / 1e12 / peak, 4)}
+    if not args.no_errors:
+        errs = rel_errors(t1m, torch, cfg, 4)
+        line["rel_err_vs_fp64"] = {p: float(f"{e:.3e}") for p, e in errs.items()}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,291 @@
+"""t1m on MI355X: transposition-free tensor contraction on gfx950.
+
+Python mirror of the reference's C++ interface (include/t1m/t1m.h):
+``contract(a, "abc", b, "bd", c, "acd")`` accumulates ``C += A . B`` over
+einsum-style labels (t1m.h:560-577), here on device tensors through the C ABI
+of ``libt1m_hip.so`` (include/t1m_hip.h).  torch supplies device memory and
+streams only; every contraction runs in the library's own HIP kernels.  There
+is no fallback: if the library is missing, import-time loading fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libt1m_hip.so")
+MAX_RANK = 16
+
+FLOAT32, FLOAT64, COMPLEX64, COMPLEX128 = 0, 1, 2, 3
+PRECISIONS = {"native": 0, "bf16x3": 1, "simple": 2}
+KERNELS = {"simple": 0, "c64_f32mfma_144x144": 1, "c64_f32mfma_64x64": 2,
+           "c128_f64mfma_64x64": 3, "c128_f64mfma_32x32": 4,
+           "c64_bf16x3_144x144": 5, "c64_bf16x3_64x64": 6}
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class TensorDesc(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("rank", ctypes.c_int32),
+                ("dims", _i64p), ("strides", _i64p), ("data", ctypes.c_void_p)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("alpha_re", ctypes.c_double), ("alpha_im", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("conj_a", ctypes.c_int32),
+                ("conj_b", ctypes.c_int32), ("synchronize", ctypes.c_int32),
+                ("stream", ctypes.c_void_p)]
+
+
+class View(ctypes.Structure):
+    _fields_ = [("br", ctypes.c_int64), ("bc", ctypes.c_int64),
+                ("rs", ctypes.c_void_p), ("nrs", ctypes.c_int64),
+                ("cs", ctypes.c_void_p), ("ncs", ctypes.c_int64),
+                ("rbs", ctypes.c_void_p), ("nrbs", ctypes.c_int64),
+                ("cbs", ctypes.c_void_p), ("ncbs", ctypes.c_int64)]
+
+
+class T1MError(RuntimeError):
+    """A failing status from libt1m_hip.so, with its message."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"t1m status {status}: {message}")
+        self.status = status
+
+
+# Exported symbols of include/t1m_hip.h.
+EXPORTS = ["t1m_options_init", "t1m_contract", "t1m_contract_vjp",
+           "t1m_batch_create", "t1m_batch_launch", "t1m_batch_destroy",
+           "t1m_index_bundles", "t1m_index_maps", "t1m_pack_block", "t1m_unpack",
+           "t1m_last_error", "t1m_version", "t1m_plan_cache_size",
+           "t1m_plan_cache_clear"]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libt1m_hip.so (after torch, so one HIP runtime serves both)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    try:
+        import torch  # noqa: F401  (binds libamdhip64.so.7 first)
+    except ImportError:
+        pass
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: build it with `make -C t1m_amd` "
+                      "or __graft_entry__.build()")
+    L = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    L.t1m_options_init.argtypes = [P(Options)]
+    L.t1m_options_init.restype = None
+    L.t1m_contract.argtypes = [P(TensorDesc), ctypes.c_char_p] * 3 + [P(Options)]
+    L.t1m_contract_vjp.argtypes = ([P(TensorDesc), ctypes.c_char_p] * 3 +
+                                   [P(TensorDesc), P(TensorDesc), P(Options)])
+    L.t1m_batch_create.argtypes = ([ctypes.c_int64] + [P(TensorDesc), ctypes.c_char_p] * 3 +
+                                   [P(Options), P(ctypes.c_void_p)])
+    L.t1m_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.t1m_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.t1m_index_bundles.argtypes = [ctypes.c_char_p] * 3 + [_i64p, _i64p]
+    L.t1m_index_maps.argtypes = [P(TensorDesc), _i64p, ctypes.c_int32, ctypes.c_int64,
+                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.t1m_pack_block.argtypes = [ctypes.c_int32, ctypes.c_int32, P(View), ctypes.c_int64,
+                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.t1m_unpack.argtypes = [ctypes.c_int32, P(View), ctypes.c_void_p, ctypes.c_void_p,
+                             ctypes.c_void_p]
+    L.t1m_last_error.restype = ctypes.c_char_p
+    L.t1m_version.restype = ctypes.c_char_p
+    L.t1m_plan_cache_size.restype = ctypes.c_int64
+    L.t1m_plan_cache_clear.restype = None
+    _lib = L
+    return L
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise T1MError(status, load().t1m_last_error().decode())
+
+
+def version() -> str:
+    return load().t1m_version().decode()
+
+
+def _dtype_code(t) -> int:
+    import torch
+    codes = {torch.float32: FLOAT32, torch.float64: FLOAT64,
+             torch.complex64: COMPLEX64, torch.complex128: COMPLEX128}
+    if t.dtype not in codes:
+        raise TypeError(f"unsupported dtype {t.dtype}")
+    return codes[t.dtype]
+
+
+class _Desc:
+    """Keeps the ctypes arrays of one tensor descriptor alive."""
+
+    def __init__(self, t, dtype: Optional[int] = None, dims=None, strides=None, ptr=None):
+        dims = list(t.shape) if dims is None else list(dims)
+        strides = list(t.stride()) if strides is None else list(strides)
+        self.dims = (ctypes.c_int64 * max(len(dims), 1))(*dims)
+        self.strides = (ctypes.c_int64 * max(len(strides), 1))(*strides)
+        self.desc = TensorDesc(_dtype_code(t) if dtype is None else dtype, len(dims),
+                               self.dims, self.strides,
+                               t.data_ptr() if ptr is None else ptr)
+
+
+def _options(alpha=1.0, precision="native", conj_a=False, conj_b=False,
+             stream=None, synchronize=True) -> Options:
+    o = Options()
+    load().t1m_options_init(ctypes.byref(o))
+    al = complex(alpha)
+    o.alpha_re, o.alpha_im = al.real, al.imag
+    o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    o.conj_a, o.conj_b = int(bool(conj_a)), int(bool(conj_b))
+    o.synchronize = int(bool(synchronize))
+    o.stream = _stream_handle(stream)
+    return o
+
+
+def _stream_handle(stream) -> Optional[int]:
+    if stream is None:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                return torch.cuda.current_stream().cuda_stream or None
+        except ImportError:
+            pass
+        return None
+    if isinstance(stream, int):
+        return stream or None
+    return stream.cuda_stream or None
+
+
+def contract(a, la: str, b, lb: str, c, lc: str, alpha=1.0, precision="native",
+             conj_a: bool = False, conj_b: bool = False, stream=None,
+             synchronize: bool = True):
+    """C += alpha * A . B over labels (t1m::contract, t1m.h:560-577).
+
+    a, b, c are torch tensors of one dtype (float32/64, complex64/128) with any
+    strides; device tensors run without copies.  Returns c."""
+    da, db, dc = _Desc(a), _Desc(b), _Desc(c)
+    o = _options(alpha, precision, conj_a, conj_b, stream, synchronize)
+    _check(load().t1m_contract(ctypes.byref(da.desc), la.encode(), ctypes.byref(db.desc),
+                               lb.encode(), ctypes.byref(dc.desc), lc.encode(),
+                               ctypes.byref(o)))
+    return c
+
+
+def contract_vjp(a, la: str, b, lb: str, grad_c, lc: str, grad_a=None, grad_b=None,
+                 alpha=1.0, precision="native", stream=None, synchronize: bool = True):
+    """grad_a += conj(alpha) grad_c . conj(B);  grad_b += conj(alpha) conj(A) . grad_c."""
+    da, db, dg = _Desc(a), _Desc(b), _Desc(grad_c)
+    ga = _Desc(grad_a) if grad_a is not None else None
+    gb = _Desc(grad_b) if grad_b is not None else None
+    o = _options(alpha, precision, False, False, stream, synchronize)
+    _check(load().t1m_contract_vjp(
+        ctypes.byref(da.desc), la.encode(), ctypes.byref(db.desc), lb.encode(),
+        ctypes.byref(dg.desc), lc.encode(),
+        ctypes.byref(ga.desc) if ga else None, ctypes.byref(gb.desc) if gb else None,
+        ctypes.byref(o)))
+    return grad_a, grad_b
+
+
+class Batch:
+    """Many contractions with one label triple, captured in one HIP graph."""
+
+    def __init__(self, a: Sequence, la: str, b: Sequence, lb: str, c: Sequence, lc: str,
+                 alpha=1.0, precision="native", conj_a=False, conj_b=False,
+                 stream=None, synchronize: bool = True):
+        n = len(a)
+        if not (len(b) == n and len(c) == n):
+            raise ValueError("a, b and c must have the same length")
+        self._keep = [(_Desc(x), _Desc(y), _Desc(z)) for x, y, z in zip(a, b, c)]
+        arr = lambda k: (TensorDesc * max(n, 1))(*[d[k].desc for d in self._keep])
+        self._a, self._b, self._c = arr(0), arr(1), arr(2)
+        o = _options(alpha, precision, conj_a, conj_b, stream, synchronize)
+        h = ctypes.c_void_p()
+        _check(load().t1m_batch_create(n, self._a, la.encode(), self._b, lb.encode(),
+                                       self._c, lc.encode(), ctypes.byref(o),
+                                       ctypes.byref(h)))
+        self._h = h
+
+    def launch(self, stream=None) -> None:
+        _check(load().t1m_batch_launch(self._h, _stream_handle(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load().t1m_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def index_bundles(la: str, lb: str, lc: str) -> dict:
+    """get_index_bundles (t1m.h:90-136); -1 marks std::string::npos."""
+    out = (ctypes.c_int64 * (6 * MAX_RANK))()
+    cnt = (ctypes.c_int64 * 6)()
+    _check(load().t1m_index_bundles(la.encode(), lb.encode(), lc.encode(), out, cnt))
+    names = ["AX", "AY", "BX", "BY", "CX", "CY"]
+    return {n: [int(out[i * MAX_RANK + j]) for j in range(cnt[i])] for i, n in enumerate(names)}
+
+
+def index_maps(dims: Sequence[int], strides: Optional[Sequence[int]], modes: Sequence[int],
+               block: int, device="cuda"):
+    """Scatter and block-scatter vectors built on the device (t1m.h:141-205).
+    Returns two int64 torch tensors holding the uint64 bit patterns."""
+    import torch
+    n = 1
+    for x in modes:
+        n *= int(dims[x])
+    scat = torch.empty(max(n, 1), dtype=torch.int64, device=device)
+    nb = -(-n // block) if block > 0 else 0
+    bscat = torch.empty(max(nb, 1), dtype=torch.int64, device=device)
+    d = (ctypes.c_int64 * max(len(dims), 1))(*dims)
+    s = (ctypes.c_int64 * max(len(dims), 1))(*(strides or [0] * len(dims)))
+    desc = TensorDesc(COMPLEX64, len(dims), d, s if strides is not None else None, None)
+    m = (ctypes.c_int64 * max(len(modes), 1))(*modes)
+    _check(load().t1m_index_maps(ctypes.byref(desc), m, len(modes), block,
+                                 scat.data_ptr(), bscat.data_ptr(), _stream_handle(None)))
+    torch.cuda.synchronize()
+    return scat[:n] if modes else scat, bscat[:nb]
+
+
+def _view(br, bc, rs, cs, rbs, cbs) -> View:
+    return View(br, bc, rs.data_ptr(), rs.numel(), cs.data_ptr(), cs.numel(),
+                rbs.data_ptr(), rbs.numel(), cbs.data_ptr(), cbs.numel())
+
+
+def pack_block(dtype: int, label: str, br: int, bc: int, rs, cs, rbs, cbs, length: int,
+               src, dest) -> None:
+    """pack_block<T, A|B> on the device (t1m.h:341-404).  rs/cs/rbs/cbs are
+    int64 device tensors (uint64 bits); dest is pre-zeroed by the caller."""
+    import torch
+    v = _view(br, bc, rs, cs, rbs, cbs)
+    _check(load().t1m_pack_block(dtype, 0 if label == "A" else 1, ctypes.byref(v), length,
+                                 src.data_ptr(), dest.data_ptr(), _stream_handle(None)))
+    torch.cuda.synchronize()
+
+
+def unpack(dtype: int, br: int, bc: int, rs, cs, rbs, cbs, src, dest) -> None:
+    """unpack on the device (t1m.h:406-430): dest[idx] += tile."""
+    import torch
+    v = _view(br, bc, rs, cs, rbs, cbs)
+    _check(load().t1m_unpack(dtype, ctypes.byref(v), src.data_ptr(), dest.data_ptr(),
+                             _stream_handle(None)))
+    torch.cuda.synchronize()
+
+
+def plan_cache_size() -> int:
+    return int(load().t1m_plan_cache_size())
+
+
+def plan_cache_clear() -> None:
+    load().t1m_plan_cache_clear()
+
+
+from .autograd import ContractFunction, contract_autograd  # noqa: E402,F401

@@ -1,0 +1,298 @@
+// complex<float> contraction in split-BF16 arithmetic ("BF16x3") on gfx950.
+//
+// Every real operand value x of the 1M GEMM (include/t1m/t1m.h:310-337) is
+// split once, while it is staged into LDS, into x_hi = bf16_rne(x) and
+// x_lo = bf16_rne(x - x_hi) (x - x_hi is exact in FP32).  Each product is
+// then a_hi*b_hi + a_hi*b_lo + a_lo*b_hi (the a_lo*b_lo term, below 2^-16
+// relative, is dropped), each term on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation (Henry, Tang & Heinecke, "Leveraging the bfloat16 artificial
+// intelligence datatype for higher-precision computations", ARITH 2019).
+//
+// Layout is the transposed 1M form of contract_mfma.hip with 32 real k per
+// MFMA: lane (j or real row c, g = lane/16) supplies 8 bf16 = 4 complex k
+// values 4g..4g+3 as (re, im) pairs.  For B (MFMA rows = columns j of C) the
+// pairs are B's own (re, im), i.e. the packed words straight from LDS.  For
+// A (MFMA columns = real rows 2i+s of C_r) the 1M expansion of A_r row 2i+s
+// is (re, -im) for s = 0 and (im, re) for s = 1 per complex k: one v_perm
+// (swap the bf16 halves) and one xor (flip the sign bit of the high half)
+// per packed word, with lane-constant selectors.  Splitting commutes with
+// negation under round-to-nearest-even, so the expanded hi/lo words are
+// exactly the splits of the expanded values.
+//
+// LDS holds one 32-bit word per complex element and plane (hi, lo), rows of
+// m (for A) / n (for B) with the k index contiguous, so a lane's fragment is
+// one 16-byte read per plane.
+#include "internal.h"
+#include "kernel_common.h"
+
+namespace t1m_hip {
+
+namespace {
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// (re, im) -> packed bf16 hi word and lo word (round to nearest even).
+__device__ inline void split_pair(float re, float im, uint32_t& hi,
+                                  uint32_t& lo) {
+  const f32x2 v = {re, im};
+  const bf16x2 h = __builtin_convertvector(v, bf16x2);
+  const f32x2 hf = __builtin_convertvector(h, f32x2);
+  const f32x2 r = {v.x - hf.x, v.y - hf.y};
+  const bf16x2 l = __builtin_convertvector(r, bf16x2);
+  hi = __builtin_bit_cast(uint32_t, h);
+  lo = __builtin_bit_cast(uint32_t, l);
+}
+
+__device__ inline f32x4 mfma_bf16(const u32x4& a, const u32x4& b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+      __builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+template <typename Idx, int BM, int BN, int BK>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(1, 1)))
+k_contract_bf16x3(KArgs<Idx> p) {
+  constexpr int MB = BM / 16;
+  constexpr int NBT = BN / 16;
+  constexpr int NBW = NBT / 2;
+  constexpr bool XT = (NBT % 2) != 0;
+  constexpr int XB = XT ? (MB + 1) / 2 : 0;
+  constexpr int XB_ALLOC = XT ? XB : 1;
+  constexpr int EA = BM * BK / 256;
+  constexpr int EB = BK * BN / 256;
+  constexpr int PITCH = BK + 4;          // words per LDS row
+  static_assert(BK % 16 == 0, "16 complex k per MFMA");
+  static_assert(BM % 16 == 0 && BN % 16 == 0, "tile shape");
+  static_assert((BM * BK) % 256 == 0 && (BK * BN) % 256 == 0, "staging");
+
+  __shared__ __attribute__((aligned(16))) uint32_t sAh[2][BM][PITCH];
+  __shared__ __attribute__((aligned(16))) uint32_t sAl[2][BM][PITCH];
+  __shared__ __attribute__((aligned(16))) uint32_t sBh[2][BN][PITCH];
+  __shared__ __attribute__((aligned(16))) uint32_t sBl[2][BN][PITCH];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int wm = wid & 1, wn = wid >> 1;
+  int tm, tn;
+  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+  const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
+  const int64_t M = p.M, N = p.N, K = p.K;
+  const float2* __restrict__ A = static_cast<const float2*>(p.A);
+  const float2* __restrict__ B = static_cast<const float2*>(p.B);
+
+  Idx aRow[EA];
+  uint32_t aOk = 0;
+#pragma unroll
+  for (int u = 0; u < EA; ++u) {
+    const int e = tid + 256 * u;
+    const int m = p.a_mfast ? e % BM : e / BK;
+    const int64_t ig = i0 + m;
+    aRow[u] = ig < M ? p.rsA[ig] : Idx(0);
+    aOk |= (ig < M ? 1u : 0u) << u;
+  }
+  Idx bCol[EB];
+  uint32_t bOk = 0;
+#pragma unroll
+  for (int u = 0; u < EB; ++u) {
+    const int e = tid + 256 * u;
+    const int n = p.b_kfast ? e / BK : e % BN;
+    const int64_t jg = j0 + n;
+    bCol[u] = jg < N ? p.csB[jg] : Idx(0);
+    bOk |= (jg < N ? 1u : 0u) << u;
+  }
+
+  float2 ra[EA], rb[EB];
+  const float sa = p.conj_a ? -1.f : 1.f;
+  const float sb = p.conj_b ? -1.f : 1.f;
+
+  auto gload = [&](int64_t p0) {
+#pragma unroll
+    for (int u = 0; u < EA; ++u) {
+      const int e = tid + 256 * u;
+      const int k = p.a_mfast ? e / BM : e % BK;
+      const int64_t kg = p0 + k;
+      float2 v = make_float2(0.f, 0.f);
+      if (((aOk >> u) & 1u) && kg < K) {
+        v = A[aRow[u] + p.csA[kg]];
+        v.y *= sa;
+      }
+      ra[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < EB; ++u) {
+      const int e = tid + 256 * u;
+      const int k = p.b_kfast ? e % BK : e / BN;
+      const int64_t kg = p0 + k;
+      float2 v = make_float2(0.f, 0.f);
+      if (((bOk >> u) & 1u) && kg < K) {
+        v = B[p.rsB[kg] + bCol[u]];
+        v.y *= sb;
+      }
+      rb[u] = v;
+    }
+  };
+  auto swrite = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < EA; ++u) {
+      const int e = tid + 256 * u;
+      const int m = p.a_mfast ? e % BM : e / BK;
+      const int k = p.a_mfast ? e / BM : e % BK;
+      uint32_t hi, lo;
+      split_pair(ra[u].x, ra[u].y, hi, lo);
+      sAh[buf][m][k] = hi;
+      sAl[buf][m][k] = lo;
+    }
+#pragma unroll
+    for (int u = 0; u < EB; ++u) {
+      const int e = tid + 256 * u;
+      const int n = p.b_kfast ? e / BK : e % BN;
+      const int k = p.b_kfast ? e % BK : e / BN;
+      uint32_t hi, lo;
+      split_pair(rb[u].x, rb[u].y, hi, lo);
+      sBh[buf][n][k] = hi;
+      sBl[buf][n][k] = lo;
+    }
+  };
+
+  f32x4 acc[MB][NBW];
+  f32x4 accx[XB_ALLOC];
+#pragma unroll
+  for (int a = 0; a < MB; ++a)
+#pragma unroll
+    for (int b = 0; b < NBW; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int x = 0; x < XB_ALLOC; ++x) accx[x] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int g = lane >> 4;
+  const int l16 = lane & 15;
+  const bool odd = (lane & 1) != 0;
+  const int am0 = wm * (BM / 2) + (l16 >> 1);
+  const int bn0 = wn * (NBW + (XT ? 1 : 0)) * 16 + l16;
+  const int bnx = NBW * 16 + l16;
+  // 1M expansion of a packed (re, im) word for real row parity s:
+  // s = 0 -> (re, -im), s = 1 -> (im, re).
+  const uint32_t psel = odd ? 0x01000302u : 0x03020100u;
+  const uint32_t pxor = odd ? 0u : 0x80000000u;
+  auto expand = [&](u32x4 w) {
+    u32x4 r;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) r[t] = __builtin_amdgcn_perm(w[t], w[t], psel) ^ pxor;
+    return r;
+  };
+
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int kc = 0; kc < BK / 16; ++kc) {
+      const int kw = kc * 16 + 4 * g;
+      u32x4 ah[MB], al[MB];
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        ah[mb] = expand(*reinterpret_cast<const u32x4*>(&sAh[buf][am0 + mb * 8][kw]));
+        al[mb] = expand(*reinterpret_cast<const u32x4*>(&sAl[buf][am0 + mb * 8][kw]));
+      }
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) {
+        const u32x4 bh = *reinterpret_cast<const u32x4*>(&sBh[buf][bn0 + nb * 16][kw]);
+        const u32x4 bl = *reinterpret_cast<const u32x4*>(&sBl[buf][bn0 + nb * 16][kw]);
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+          acc[mb][nb] = mfma_bf16(bl, ah[mb], acc[mb][nb]);
+          acc[mb][nb] = mfma_bf16(bh, al[mb], acc[mb][nb]);
+          acc[mb][nb] = mfma_bf16(bh, ah[mb], acc[mb][nb]);
+        }
+      }
+      if constexpr (XT) {
+        const u32x4 bh = *reinterpret_cast<const u32x4*>(&sBh[buf][bnx][kw]);
+        const u32x4 bl = *reinterpret_cast<const u32x4*>(&sBl[buf][bnx][kw]);
+        if (wn == 0) {
+#pragma unroll
+          for (int x = 0; x < XB; ++x) {
+            accx[x] = mfma_bf16(bl, ah[x], accx[x]);
+            accx[x] = mfma_bf16(bh, al[x], accx[x]);
+            accx[x] = mfma_bf16(bh, ah[x], accx[x]);
+          }
+        } else {
+#pragma unroll
+          for (int x = 0; x < XB; ++x) {
+            constexpr int kLast = MB - 1;
+            const int mb = (XB + x) < MB ? (XB + x) : kLast;
+            accx[x] = mfma_bf16(bl, ah[mb], accx[x]);
+            accx[x] = mfma_bf16(bh, al[mb], accx[x]);
+            accx[x] = mfma_bf16(bh, ah[mb], accx[x]);
+          }
+        }
+      }
+    }
+  };
+
+  const int nst = int((K + BK - 1) / BK);
+  gload(0);
+  swrite(0);
+  __syncthreads();
+  for (int s = 0; s < nst; ++s) {
+    const bool more = s + 1 < nst;
+    if (more) gload(int64_t(s + 1) * BK);
+    compute(s & 1);
+    if (more) swrite((s + 1) & 1);
+    __syncthreads();
+  }
+
+  float* __restrict__ Cr = static_cast<float*>(p.C);
+  const bool cplx_alpha = p.alpha_im != 0.0;
+  const float ar = float(p.alpha_re), ai = float(p.alpha_im);
+  auto store = [&](const f32x4& v, int64_t i, int64_t jb) {
+    const bool iok = i < M;
+    const Idx rc = iok ? p.rsC[i] : Idx(0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float val = v[r];
+      if (cplx_alpha) {
+        const float other = __shfl_xor(val, 1);
+        val = odd ? ar * val + ai * other : ar * val - ai * other;
+      } else {
+        val = ar * val;
+      }
+      const int64_t j = jb + r;
+      if (iok && j < N) Cr[2 * int64_t(rc + p.csC[j]) + (odd ? 1 : 0)] += val;
+    }
+  };
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) {
+    const int64_t i = i0 + am0 + mb * 8;
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb)
+      store(acc[mb][nb], i, j0 + wn * (NBW + (XT ? 1 : 0)) * 16 + nb * 16 + g * 4);
+  }
+  if constexpr (XT) {
+#pragma unroll
+    for (int x = 0; x < XB; ++x) {
+      const int mb = wn == 0 ? x : XB + x;
+      if (mb < MB) store(accx[x], i0 + am0 + mb * 8, j0 + NBW * 16 + g * 4);
+    }
+  }
+}
+
+template <typename Idx, int BM, int BN, int BK>
+hipError_t launch_t(const GemmProblem& g, hipStream_t s) {
+  const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
+  const dim3 grid(unsigned(a.tiles_m * a.tiles_n)), block(256);
+  hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename Idx>
+hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
+  if (kernel == K_C64_BF16X3_144) return launch_t<Idx, 144, 144, 16>(g, s);
+  return launch_t<Idx, 64, 64, 16>(g, s);
+}
+
+}  // namespace
+
+hipError_t launch_bf16x3(const GemmProblem& g, int32_t kernel, hipStream_t s) {
+  return g.idx32 ? dispatch<int32_t>(g, kernel, s) : dispatch<int64_t>(g, kernel, s);
+}
+
+}  // namespace t1m_hip

@@ -1,0 +1,355 @@
+// Transposition-free complex contraction on gfx950 matrix cores (1M method).
+//
+// The reference executes a complex contraction as a real GEMM on 1M panels:
+// A becomes 2x2 real blocks [re -im; im re] and B becomes [re; im]
+// (include/t1m/t1m.h:310-337), so C_r (2m x n) = A_r (2m x 2k) . B_r (2k x n)
+// costs exactly 8mnk real flops (t1m.h:508-554, BLIS ukernel at 546-547).
+//
+// Here one workgroup owns a BM x BN tile of complex C and walks the whole K
+// range (no packing pass, no workspace):
+//   * gather: each lane loads complex A / B elements straight from strided
+//     HBM through the scatter vectors (rs/cs as in t1m.h:251-252), lanes
+//     along the operand's unit-stride direction, and stores them to LDS as
+//     (re, im) pairs -- the real-valued panels of 1M.  Two LDS buffers: the
+//     global loads of stage s+1 are in flight while stage s is multiplied.
+//   * multiply: v_mfma_f32_16x16x4f32 (FP32) or v_mfma_f64_16x16x4f64 (FP64)
+//     in the transposed 1M form: MFMA rows are 16 columns j of C, MFMA columns
+//     are 16 real rows of C_r (8 complex rows i, re/im interleaved).  Lane
+//     (g = lane/16) reads one complex B[q_g][j] and one complex A[i][q_g]; the
+//     first MFMA uses B's real parts with A_r's even real columns
+//     (s ? im : re), the second B's imaginary parts with A_r's odd real
+//     columns (s ? re : -im), s = row parity.  Two MFMAs therefore cover four
+//     complex k and the 1M expansion of A never leaves registers.
+//   * scatter epilogue: lane l holds re (even l) or im (odd l) of
+//     C[i][j]; C is read-modified-written through rsC/csC (C += alpha*A.B,
+//     t1m.h:543-548), so arbitrarily strided C works and rows with unit stride
+//     give contiguous 64-byte runs per 16 lanes.
+//
+// Wave layout: 4 waves as 2 (m) x 2 (n).  A wave owns BM/2 complex rows,
+// i.e. MB = BM/16 blocks of 8 rows, and BN/2 columns.  When BN/16 is odd the
+// middle 16-column block is shared: each n-wave takes half of its row blocks
+// (the 144 x 144 tile: 9 blocks of 16 columns, waves own 4 + half of the
+// middle one, 41 MFMA tiles each instead of 45 for a padded split).
+#include "internal.h"
+#include "kernel_common.h"
+
+namespace t1m_hip {
+
+namespace {
+
+template <typename T, typename Idx, int BM, int BN, int BK>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(1, 1)))
+k_contract_1m(KArgs<Idx> p) {
+  using C2 = typename Vec<T>::c2;
+  using Acc = typename Vec<T>::acc;
+  constexpr int MB = BM / 16;            // 8-row blocks per wave
+  constexpr int NBT = BN / 16;           // 16-column blocks per workgroup
+  constexpr int NBW = NBT / 2;           // full column blocks per wave
+  constexpr bool XT = (NBT % 2) != 0;    // shared middle column block
+  constexpr int XB = XT ? (MB + 1) / 2 : 0;
+  constexpr int XB_ALLOC = XT ? XB : 1;
+  constexpr int EA = BM * BK / 256;      // A elements staged per lane
+  constexpr int EB = BK * BN / 256;
+  // Row pitch padding (complex elements) so the four lane groups of a
+  // fragment read hit different LDS banks.
+  constexpr int PADA = sizeof(T) == 4 ? 8 : 4;
+  constexpr int PADB = sizeof(T) == 4 ? 8 : 4;
+  constexpr int LDA = BM + PADA, LDB = BN + PADB;
+  static_assert(BM % 16 == 0 && BN % 16 == 0, "tile shape");
+  static_assert((BM * BK) % 256 == 0 && (BK * BN) % 256 == 0, "staging");
+  static_assert(BK % 4 == 0, "k chunk");
+
+  __shared__ C2 sA[2][BK][LDA];
+  __shared__ C2 sB[2][BK][LDB];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int wm = wid & 1, wn = wid >> 1;
+  int tm, tn;
+  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+  const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
+  const int64_t M = p.M, N = p.N, K = p.K;
+
+  const C2* __restrict__ A = static_cast<const C2*>(p.A);
+  const C2* __restrict__ B = static_cast<const C2*>(p.B);
+
+  // ---- per-lane staging coordinates (fixed for the whole K loop) --------
+  Idx aRow[EA];
+  uint32_t aOk = 0;
+#pragma unroll
+  for (int u = 0; u < EA; ++u) {
+    const int e = tid + 256 * u;
+    const int m = p.a_mfast ? e % BM : e / BK;
+    const int64_t ig = i0 + m;
+    aRow[u] = ig < M ? p.rsA[ig] : Idx(0);
+    aOk |= (ig < M ? 1u : 0u) << u;
+  }
+  Idx bCol[EB];
+  uint32_t bOk = 0;
+#pragma unroll
+  for (int u = 0; u < EB; ++u) {
+    const int e = tid + 256 * u;
+    const int n = p.b_kfast ? e / BK : e % BN;
+    const int64_t jg = j0 + n;
+    bCol[u] = jg < N ? p.csB[jg] : Idx(0);
+    bOk |= (jg < N ? 1u : 0u) << u;
+  }
+
+  C2 ra[EA], rb[EB];
+  const T sa = p.conj_a ? T(-1) : T(1);
+  const T sb = p.conj_b ? T(-1) : T(1);
+
+  auto gload = [&](int64_t p0) {
+#pragma unroll
+    for (int u = 0; u < EA; ++u) {
+      const int e = tid + 256 * u;
+      const int k = p.a_mfast ? e / BM : e % BK;
+      const int64_t kg = p0 + k;
+      C2 v;
+      v.x = T(0); v.y = T(0);
+      if (((aOk >> u) & 1u) && kg < K) {
+        v = A[aRow[u] + p.csA[kg]];
+        v.y *= sa;
+      }
+      ra[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < EB; ++u) {
+      const int e = tid + 256 * u;
+      const int k = p.b_kfast ? e % BK : e / BN;
+      const int64_t kg = p0 + k;
+      C2 v;
+      v.x = T(0); v.y = T(0);
+      if (((bOk >> u) & 1u) && kg < K) {
+        v = B[p.rsB[kg] + bCol[u]];
+        v.y *= sb;
+      }
+      rb[u] = v;
+    }
+  };
+  auto swrite = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < EA; ++u) {
+      const int e = tid + 256 * u;
+      const int m = p.a_mfast ? e % BM : e / BK;
+      const int k = p.a_mfast ? e / BM : e % BK;
+      sA[buf][k][m] = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < EB; ++u) {
+      const int e = tid + 256 * u;
+      const int n = p.b_kfast ? e / BK : e % BN;
+      const int k = p.b_kfast ? e % BK : e / BN;
+      sB[buf][k][n] = rb[u];
+    }
+  };
+
+  Acc acc[MB][NBW];
+  Acc accx[XB_ALLOC];
+#pragma unroll
+  for (int a = 0; a < MB; ++a)
+#pragma unroll
+    for (int b = 0; b < NBW; ++b) acc[a][b] = Acc{0, 0, 0, 0};
+#pragma unroll
+  for (int x = 0; x < XB_ALLOC; ++x) accx[x] = Acc{0, 0, 0, 0};
+
+  const int g = lane >> 4;           // k sub-index inside a 4-k chunk
+  const int l16 = lane & 15;
+  const bool odd = (lane & 1) != 0;  // real row parity (re / im of C)
+  const int am0 = wm * (BM / 2) + (l16 >> 1);
+  const int bn0 = wn * (NBW + (XT ? 1 : 0)) * 16 + l16;
+  const int bnx = NBW * 16 + l16;
+
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int kc = 0; kc < BK / 4; ++kc) {
+      const int q = kc * 4 + g;
+      T are[MB], aim[MB];
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const C2 a = sA[buf][q][am0 + mb * 8];
+        are[mb] = odd ? a.y : a.x;     // A_r[2i+s][2q]
+        aim[mb] = odd ? a.x : -a.y;    // A_r[2i+s][2q+1]
+      }
+      C2 bf[NBW];
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) bf[nb] = sB[buf][q][bn0 + nb * 16];
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) {
+          acc[mb][nb] = mfma16(bf[nb].x, are[mb], acc[mb][nb]);
+          acc[mb][nb] = mfma16(bf[nb].y, aim[mb], acc[mb][nb]);
+        }
+      if constexpr (XT) {
+        const C2 bx = sB[buf][q][bnx];
+        if (wn == 0) {
+#pragma unroll
+          for (int x = 0; x < XB; ++x) {
+            accx[x] = mfma16(bx.x, are[x], accx[x]);
+            accx[x] = mfma16(bx.y, aim[x], accx[x]);
+          }
+        } else {
+#pragma unroll
+          for (int x = 0; x < XB; ++x) {
+            constexpr int kLast = MB - 1;
+            const int mb = (XB + x) < MB ? (XB + x) : kLast;
+            accx[x] = mfma16(bx.x, are[mb], accx[x]);
+            accx[x] = mfma16(bx.y, aim[mb], accx[x]);
+          }
+        }
+      }
+    }
+  };
+
+  // ---- main loop: double-buffered LDS, one barrier per stage ------------
+  const int nst = int((K + BK - 1) / BK);
+  gload(0);
+  swrite(0);
+  __syncthreads();
+  for (int s = 0; s < nst; ++s) {
+    const bool more = s + 1 < nst;
+    if (more) gload(int64_t(s + 1) * BK);
+    compute(s & 1);
+    if (more) swrite((s + 1) & 1);
+    __syncthreads();
+  }
+
+  // ---- epilogue: C[rsC[i] + csC[j]] (+)= alpha * acc ---------------------
+  T* __restrict__ Cr = static_cast<T*>(p.C);
+  const bool cplx_alpha = p.alpha_im != 0.0;
+  const T ar = T(p.alpha_re), ai = T(p.alpha_im);
+  auto store = [&](const Acc& v, int64_t i, int64_t jb) {
+    const bool iok = i < M;
+    const Idx rc = iok ? p.rsC[i] : Idx(0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      T val = v[r];
+      if (cplx_alpha) {
+        const T other = __shfl_xor(val, 1);
+        val = odd ? ar * val + ai * other : ar * val - ai * other;
+      } else {
+        val = ar * val;
+      }
+      const int64_t j = jb + r;
+      if (iok && j < N) {
+        T* dst = Cr + 2 * int64_t(rc + p.csC[j]) + (odd ? 1 : 0);
+        *dst += val;
+      }
+    }
+  };
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) {
+    const int64_t i = i0 + am0 + mb * 8;
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb)
+      store(acc[mb][nb], i, j0 + wn * (NBW + (XT ? 1 : 0)) * 16 + nb * 16 + g * 4);
+  }
+  if constexpr (XT) {
+#pragma unroll
+    for (int x = 0; x < XB; ++x) {
+      const int mb = wn == 0 ? x : XB + x;
+      if (mb < MB)
+        store(accx[x], i0 + am0 + mb * 8, j0 + NBW * 16 + g * 4);
+    }
+  }
+}
+
+// One lane per output element: a plain loop over the contracted index.
+// Works for every dtype; used as the cross-check path and for problems the
+// MFMA tiles do not cover (real dtypes).
+template <typename T, bool kComplex, typename Idx>
+__global__ void __launch_bounds__(256) k_contract_simple(KArgs<Idx> p) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= p.M * p.N) return;
+  const int64_t i = e % p.M, j = e / p.M;
+  const Idx ra = p.rsA[i], cb = p.csB[j];
+  const T* A = static_cast<const T*>(p.A);
+  const T* B = static_cast<const T*>(p.B);
+  T* C = static_cast<T*>(p.C);
+  if constexpr (kComplex) {
+    const T sa = p.conj_a ? T(-1) : T(1), sb = p.conj_b ? T(-1) : T(1);
+    T re = 0, im = 0;
+    for (int64_t q = 0; q < p.K; ++q) {
+      const int64_t oa = 2 * int64_t(ra + p.csA[q]);
+      const int64_t ob = 2 * int64_t(p.rsB[q] + cb);
+      const T xr = A[oa], xi = sa * A[oa + 1];
+      const T yr = B[ob], yi = sb * B[ob + 1];
+      re = fma(xr, yr, re);
+      re = fma(-xi, yi, re);
+      im = fma(xr, yi, im);
+      im = fma(xi, yr, im);
+    }
+    const T ar = T(p.alpha_re), ai = T(p.alpha_im);
+    const int64_t oc = 2 * int64_t(p.rsC[i] + p.csC[j]);
+    C[oc] += ar * re - ai * im;
+    C[oc + 1] += ar * im + ai * re;
+  } else {
+    T acc = 0;
+    for (int64_t q = 0; q < p.K; ++q)
+      acc = fma(A[ra + p.csA[q]], B[p.rsB[q] + cb], acc);
+    C[p.rsC[i] + p.csC[j]] += T(p.alpha_re) * acc;
+  }
+}
+
+template <typename Idx>
+hipError_t launch_simple(const GemmProblem& g, hipStream_t s) {
+  const KArgs<Idx> a = make_args<Idx>(g, 0, 0);
+  const int64_t n = g.M * g.N;
+  const dim3 grid(unsigned((n + 255) / 256)), block(256);
+  switch (g.dtype) {
+    case T1M_FLOAT32: hipLaunchKernelGGL((k_contract_simple<float, false, Idx>), grid, block, 0, s, a); break;
+    case T1M_FLOAT64: hipLaunchKernelGGL((k_contract_simple<double, false, Idx>), grid, block, 0, s, a); break;
+    case T1M_COMPLEX64: hipLaunchKernelGGL((k_contract_simple<float, true, Idx>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_contract_simple<double, true, Idx>), grid, block, 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+template <typename T, typename Idx, int BM, int BN, int BK>
+hipError_t launch_1m(const GemmProblem& g, hipStream_t s) {
+  const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
+  const dim3 grid(unsigned(a.tiles_m * a.tiles_n)), block(256);
+  hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename Idx>
+hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
+  switch (kernel) {
+    case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, 16>(g, s);
+    case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 16>(g, s);
+    case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16>(g, s);
+    case K_C128_F64_32: return launch_1m<double, Idx, 32, 32, 16>(g, s);
+    default: return launch_simple<Idx>(g, s);
+  }
+}
+
+const KernelInfo kInfo[K_COUNT] = {
+    {"simple", 0, 0},
+    {"c64_f32mfma_144x144", 144, 144},
+    {"c64_f32mfma_64x64", 64, 64},
+    {"c128_f64mfma_64x64", 64, 64},
+    {"c128_f64mfma_32x32", 32, 32},
+    {"c64_bf16x3_144x144", 144, 144},
+    {"c64_bf16x3_64x64", 64, 64},
+};
+
+}  // namespace
+
+hipError_t launch_bf16x3(const GemmProblem& g, int32_t kernel, hipStream_t s);
+
+const KernelInfo& kernel_info(int32_t id) {
+  return kInfo[(id >= 0 && id < K_COUNT) ? id : 0];
+}
+
+hipError_t launch_contract(const GemmProblem& g, int32_t kernel,
+                           hipStream_t s) {
+  if (g.M == 0 || g.N == 0) return hipSuccess;
+  if (kernel == K_C64_BF16X3_144 || kernel == K_C64_BF16X3_64)
+    return launch_bf16x3(g, kernel, s);
+  return g.idx32 ? dispatch<int32_t>(g, kernel, s) : dispatch<int64_t>(g, kernel, s);
+}
+
+}  // namespace t1m_hip

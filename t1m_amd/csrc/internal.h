@@ -1,0 +1,86 @@
+// Internal declarations shared by the translation units of libt1m_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "t1m_hip.h"
+
+namespace t1m_hip {
+
+// ---- index maps ---------------------------------------------------------
+
+// One scatter vector to build: entry e = sum_j (e_j) * strides[j] with the
+// mixed-radix digits e_j of e over dims[] (first mode fastest).
+struct MapDesc {
+  int32_t nmodes;
+  int32_t out32;          // 1: write int32, 0: write 64-bit words
+  int64_t len;            // product of dims (1 for no modes)
+  int64_t dims[T1M_MAX_RANK];
+  int64_t strides[T1M_MAX_RANK];
+  void* out;
+};
+
+constexpr int kMaxMapsPerLaunch = 6;
+struct MapBatch {
+  int32_t count;
+  MapDesc maps[kMaxMapsPerLaunch];
+};
+
+// Builds up to six scatter vectors in one launch (grid.y = map).
+hipError_t launch_scatter_maps(const MapBatch& batch, hipStream_t stream);
+// get_block_scatter on the device, one lane per block.
+hipError_t launch_block_scatter(const uint64_t* scat, int64_t len, int64_t b,
+                                uint64_t* out, hipStream_t stream);
+hipError_t launch_pack_block(int dtype, int label, const t1m_view& v,
+                             int64_t length, const void* src, void* dest,
+                             hipStream_t stream);
+hipError_t launch_unpack(int dtype, const t1m_view& v, const void* src,
+                         void* dest, hipStream_t stream);
+
+// ---- contraction kernels ------------------------------------------------
+
+// A matricised contraction: C[rsC[i] + csC[j]] += alpha * sum_p
+// A[rsA[i] + csA[p]] * B[rsB[p] + csB[j]], offsets in elements.
+struct GemmProblem {
+  const void* A;
+  const void* B;
+  void* C;
+  const void* rsA;   // M entries (int32 or int64, see idx32)
+  const void* csA;   // K
+  const void* rsB;   // K
+  const void* csB;   // N
+  const void* rsC;   // M
+  const void* csC;   // N
+  int64_t M, N, K;
+  double alpha_re, alpha_im;
+  int32_t conj_a, conj_b;
+  int32_t a_mfast;   // stage A with lanes along m (else along k)
+  int32_t b_kfast;   // stage B with lanes along k (else along n)
+  int32_t idx32;     // offsets are int32 (else int64)
+  int32_t dtype;     // t1m_dtype
+};
+
+// Kernel families.
+enum KernelId : int32_t {
+  K_SIMPLE = 0,        // one lane per output element (any dtype)
+  K_C64_F32_144 = 1,   // complex<float>, 1M on FP32 MFMA, 144x144 tiles
+  K_C64_F32_64 = 2,    // complex<float>, 1M on FP32 MFMA, 64x64 tiles
+  K_C128_F64_64 = 3,   // complex<double>, 1M on FP64 MFMA, 64x64 tiles
+  K_C128_F64_32 = 4,   // complex<double>, 1M on FP64 MFMA, 32x32 tiles
+  K_C64_BF16X3_144 = 5,// complex<float>, 1M BF16x3 MFMA, 144x144 tiles
+  K_C64_BF16X3_64 = 6, // complex<float>, 1M BF16x3 MFMA, 64x64 tiles
+  K_COUNT
+};
+
+struct KernelInfo {
+  const char* name;
+  int32_t bm, bn;      // workgroup tile, in complex (or real) outputs
+};
+const KernelInfo& kernel_info(int32_t id);
+
+hipError_t launch_contract(const GemmProblem& p, int32_t kernel,
+                           hipStream_t stream);
+
+}  // namespace t1m_hip

@@ -1,0 +1,80 @@
+// Device-side helpers shared by the contraction kernels.
+#pragma once
+
+#include "internal.h"
+
+namespace t1m_hip {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Vec;
+template <> struct Vec<float> { using c2 = float2; using acc = f32x4; };
+template <> struct Vec<double> { using c2 = double2; using acc = f64x4; };
+
+__device__ inline f32x4 mfma16(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ inline f64x4 mfma16(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename Idx>
+struct KArgs {
+  const void* A;
+  const void* B;
+  void* C;
+  const Idx* rsA;
+  const Idx* csA;
+  const Idx* rsB;
+  const Idx* csB;
+  const Idx* rsC;
+  const Idx* csC;
+  int64_t M, N, K;
+  double alpha_re, alpha_im;
+  int32_t conj_a, conj_b;
+  int32_t a_mfast, b_kfast;
+  int32_t tiles_m, tiles_n;
+};
+
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
+// Renumber so that XCD x receives a contiguous range of logical tiles, and
+// order logical tiles in bands of kGroupM tile-rows (column-major inside a
+// band) so a range covers a compact 2-D block of C that shares A row panels
+// and B column panels.
+constexpr int kXcds = 8;
+constexpr int kGroupM = 4;
+
+__device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
+  const int nwg = tiles_m * tiles_n;
+  int w = blockIdx.x;
+  if (nwg % kXcds == 0) w = (w % kXcds) * (nwg / kXcds) + w / kXcds;
+  const int band = kGroupM * tiles_n;
+  const int b0 = w / band * kGroupM;
+  const int rows = (tiles_m - b0) < kGroupM ? (tiles_m - b0) : kGroupM;
+  const int r = w % band;
+  tm = b0 + r % rows;
+  tn = r / rows;
+}
+
+
+template <typename Idx>
+KArgs<Idx> make_args(const GemmProblem& g, int bm, int bn) {
+  KArgs<Idx> a;
+  a.A = g.A; a.B = g.B; a.C = g.C;
+  a.rsA = static_cast<const Idx*>(g.rsA);
+  a.csA = static_cast<const Idx*>(g.csA);
+  a.rsB = static_cast<const Idx*>(g.rsB);
+  a.csB = static_cast<const Idx*>(g.csB);
+  a.rsC = static_cast<const Idx*>(g.rsC);
+  a.csC = static_cast<const Idx*>(g.csC);
+  a.M = g.M; a.N = g.N; a.K = g.K;
+  a.alpha_re = g.alpha_re; a.alpha_im = g.alpha_im;
+  a.conj_a = g.conj_a; a.conj_b = g.conj_b;
+  a.a_mfast = g.a_mfast; a.b_kfast = g.b_kfast;
+  a.tiles_m = bm > 0 ? int((g.M + bm - 1) / bm) : 0;
+  a.tiles_n = bn > 0 ? int((g.N + bn - 1) / bn) : 0;
+  return a;
+}
+
+}  // namespace t1m_hip

@@ -1,0 +1,86 @@
+// Host-side runtime declarations (plans, problems, staging, errors).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace t1m_hip {
+
+int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+const char* last_error();
+
+#define HIP_TRY(expr)                                                         \
+  do {                                                                        \
+    const hipError_t e_ = (expr);                                             \
+    if (e_ != hipSuccess)                                                     \
+      return ::t1m_hip::set_error(T1M_ERROR_DEVICE, "%s: %s", #expr,          \
+                                  hipGetErrorString(e_));                     \
+  } while (0)
+
+// One mode (label) of the contraction with its extent and its stride in
+// each operand (0 where the operand lacks the mode).
+struct Mode {
+  char label;
+  int64_t dim;
+  int64_t sa, sb, sc;
+};
+
+// A validated contraction: free modes of A (m), of B (n), contracted (k),
+// each in the reference's alphabetical bundle order (t1m.h:99-133).
+struct Problem {
+  int dtype = 0;
+  std::string la, lb, lc;
+  std::vector<Mode> m, n, k;
+  int64_t M = 1, N = 1, K = 1;
+  std::vector<int64_t> da, sa, db, sb, dc, sc;
+};
+
+// Device-resident index maps for one (dtype, labels, extents, strides).
+struct Plan {
+  int dtype = 0;
+  int kernel = 0;
+  int64_t M = 0, N = 0, K = 0;
+  int a_mfast = 1, b_kfast = 1, idx32 = 1;
+  void* maps = nullptr;
+  const void *rsA = nullptr, *rsC = nullptr, *csA = nullptr, *rsB = nullptr,
+             *csB = nullptr, *csC = nullptr;
+  std::vector<Mode> modes_m, modes_n, modes_k;   // kernel traversal order
+  ~Plan();
+};
+
+// A host tensor copied through device memory for one call.
+struct Staged {
+  void* buf = nullptr;
+  void* device = nullptr;        // element-0 address inside buf
+  const char* host_base = nullptr;
+  size_t bytes = 0;
+  int stage(const void* host, const std::vector<int64_t>& dims,
+            const std::vector<int64_t>& strides, int esize, bool copy_in,
+            hipStream_t stream);
+  int copy_back(void* host, hipStream_t stream);
+  ~Staged();
+};
+
+int dtype_size(int dtype);
+int check_tensor(const t1m_tensor* t, const char* labels, const char* what,
+                 std::vector<int64_t>& strides);
+int parse_problem(const t1m_tensor* a, const char* la, const t1m_tensor* b,
+                  const char* lb, const t1m_tensor* c, const char* lc,
+                  Problem& P);
+int choose_kernel(const Problem& P, int precision);
+int get_plan(const Problem& P, int precision, hipStream_t stream,
+             std::shared_ptr<Plan>& out);
+int64_t plan_cache_size();
+void plan_cache_clear();
+bool is_device_pointer(const void* ptr);
+int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
+                  const char* lb, const t1m_tensor* c, const char* lc,
+                  const t1m_options& opt);
+
+}  // namespace t1m_hip

@@ -1,0 +1,577 @@
+// Host runtime of libt1m_hip.so: the extern "C" entry points of t1m_hip.h.
+//
+// t1m_contract follows t1m::internal::contract (include/t1m/t1m.h:432-557):
+//   1. index bundles from the labels (t1m.h:440-441 -> 90-136), validated;
+//   2. scatter vectors for the rows / columns of A, B and C (t1m.h:442-444 ->
+//      246-268), built on the device by one kernel launch and cached in a
+//      plan keyed by dtype, labels, extents and strides;
+//   3. one fused contraction kernel (gather -> MFMA -> scatter-accumulate),
+//      in place of the reference's pack / micro-kernel / unpack loops.
+// Where the reference keeps the bundle order of t1m.h:99-133 (alphabetical)
+// for the traversal, the kernels here may traverse a bundle's modes in
+// another order (a sum over K or an enumeration of M / N is order-free):
+// each bundle is ordered so that the operand staged along it has its
+// smallest stride first, which makes the gathers coalesce.  The exported
+// index maps (t1m_index_maps) keep the reference order bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "internal.h"
+#include "runtime.h"
+
+namespace t1m_hip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+void clear_error() { g_last_error.clear(); }
+const char* last_error() { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Labels -> modes
+// ---------------------------------------------------------------------------
+
+int dtype_size(int dtype) {
+  switch (dtype) {
+    case T1M_FLOAT32: return 4;
+    case T1M_FLOAT64: return 8;
+    case T1M_COMPLEX64: return 8;
+    case T1M_COMPLEX128: return 16;
+    default: return 0;
+  }
+}
+
+int check_tensor(const t1m_tensor* t, const char* labels, const char* what,
+                 std::vector<int64_t>& strides) {
+  if (!t) return set_error(T1M_ERROR_INVALID_ARGUMENT, "%s: null tensor", what);
+  if (!labels) return set_error(T1M_ERROR_INVALID_ARGUMENT, "%s: null labels", what);
+  if (dtype_size(t->dtype) == 0)
+    return set_error(T1M_ERROR_INVALID_ARGUMENT, "%s: unknown dtype %d", what, t->dtype);
+  if (t->rank < 0 || t->rank > T1M_MAX_RANK)
+    return set_error(T1M_ERROR_INVALID_ARGUMENT, "%s: rank %d out of range [0, %d]",
+                     what, t->rank, T1M_MAX_RANK);
+  if (int(strlen(labels)) != t->rank)
+    return set_error(T1M_ERROR_INVALID_ARGUMENT,
+                     "%s: %d labels \"%s\" for a rank-%d tensor", what,
+                     int(strlen(labels)), labels, t->rank);
+  if (t->rank > 0 && !t->dims)
+    return set_error(T1M_ERROR_INVALID_ARGUMENT, "%s: null dims", what);
+  for (int i = 0; i < t->rank; ++i) {
+    if (t->dims[i] < 0)
+      return set_error(T1M_ERROR_INVALID_ARGUMENT, "%s: negative extent", what);
+    for (int j = 0; j < i; ++j)
+      if (labels[i] == labels[j])
+        return set_error(T1M_ERROR_INVALID_ARGUMENT,
+                         "%s: label '%c' repeated in \"%s\"", what, labels[i], labels);
+  }
+  strides.assign(t->rank, 0);
+  if (t->strides) {
+    for (int i = 0; i < t->rank; ++i) strides[i] = t->strides[i];
+  } else {
+    int64_t acc = 1;   // column-major, t1m.h:42-50
+    for (int i = 0; i < t->rank; ++i) { strides[i] = acc; acc *= t->dims[i]; }
+  }
+  return T1M_SUCCESS;
+}
+
+int parse_problem(const t1m_tensor* a, const char* la, const t1m_tensor* b,
+                  const char* lb, const t1m_tensor* c, const char* lc,
+                  Problem& P) {
+  std::vector<int64_t> sa, sb, sc;
+  int rc;
+  if ((rc = check_tensor(a, la, "A", sa))) return rc;
+  if ((rc = check_tensor(b, lb, "B", sb))) return rc;
+  if ((rc = check_tensor(c, lc, "C", sc))) return rc;
+  if (a->dtype != b->dtype || a->dtype != c->dtype)
+    return set_error(T1M_ERROR_INVALID_ARGUMENT, "A, B and C must share one dtype");
+  P.dtype = a->dtype;
+  P.la = la; P.lb = lb; P.lc = lc;
+  P.m.clear(); P.n.clear(); P.k.clear();
+  auto pos = [](const char* s, char ch) -> int {
+    const char* q = strchr(s, ch);
+    return (q && ch) ? int(q - s) : -1;
+  };
+  // Contracted and free modes, alphabetical (t1m.h:99-111).
+  std::string sorted_a(la), sorted_b(lb);
+  std::sort(sorted_a.begin(), sorted_a.end());
+  std::sort(sorted_b.begin(), sorted_b.end());
+  for (char ch : sorted_a) {
+    const int pa = pos(la, ch), pb = pos(lb, ch), pc = pos(lc, ch);
+    if (pb >= 0) {
+      if (pc >= 0)
+        return set_error(T1M_ERROR_UNSUPPORTED,
+                         "label '%c' appears in A, B and C (batch labels are not supported)", ch);
+      if (a->dims[pa] != b->dims[pb])
+        return set_error(T1M_ERROR_INVALID_ARGUMENT,
+                         "extent mismatch for contracted label '%c': %lld vs %lld", ch,
+                         (long long)a->dims[pa], (long long)b->dims[pb]);
+      P.k.push_back({ch, a->dims[pa], sa[pa], sb[pb], 0});
+    } else {
+      if (pc < 0)
+        return set_error(T1M_ERROR_INVALID_ARGUMENT,
+                         "label '%c' of A is neither in B nor in C", ch);
+      if (a->dims[pa] != c->dims[pc])
+        return set_error(T1M_ERROR_INVALID_ARGUMENT,
+                         "extent mismatch for label '%c' of A and C", ch);
+      P.m.push_back({ch, a->dims[pa], sa[pa], 0, sc[pc]});
+    }
+  }
+  for (char ch : sorted_b) {
+    const int pb = pos(lb, ch), pc = pos(lc, ch);
+    if (pos(la, ch) >= 0) continue;
+    if (pc < 0)
+      return set_error(T1M_ERROR_INVALID_ARGUMENT,
+                       "label '%c' of B is neither in A nor in C", ch);
+    if (b->dims[pb] != c->dims[pc])
+      return set_error(T1M_ERROR_INVALID_ARGUMENT,
+                       "extent mismatch for label '%c' of B and C", ch);
+    P.n.push_back({ch, b->dims[pb], 0, sb[pb], sc[pc]});
+  }
+  for (const char* q = lc; *q; ++q)
+    if (pos(la, *q) < 0 && pos(lb, *q) < 0)
+      return set_error(T1M_ERROR_INVALID_ARGUMENT,
+                       "label '%c' of C is in neither A nor B", *q);
+  P.M = P.N = P.K = 1;
+  for (auto& x : P.m) P.M *= x.dim;
+  for (auto& x : P.n) P.N *= x.dim;
+  for (auto& x : P.k) P.K *= x.dim;
+  P.sa = sa; P.sb = sb; P.sc = sc;
+  P.da.assign(a->dims, a->dims + a->rank);
+  P.db.assign(b->dims, b->dims + b->rank);
+  P.dc.assign(c->dims, c->dims + c->rank);
+  return T1M_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel choice
+// ---------------------------------------------------------------------------
+
+namespace {
+
+int cu_count() {
+  static int n = [] {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 256;
+    return cus > 0 ? cus : 256;
+  }();
+  return n;
+}
+
+// Relative cost of running an M x N (x K) problem with BM x BN tiles:
+// rounds of workgroups times per-tile work over the tile's expected MFMA
+// efficiency.  Resident workgroups per CU = occ.
+double tile_cost(int64_t M, int64_t N, int bm, int bn, int occ, double eff) {
+  const int64_t tiles = ((M + bm - 1) / bm) * ((N + bn - 1) / bn);
+  const int64_t slots = int64_t(cu_count()) * occ;
+  const int64_t rounds = (tiles + slots - 1) / slots;
+  return double(rounds) * double(bm) * double(bn) / (eff * occ);
+}
+
+}  // namespace
+
+int choose_kernel(const Problem& P, int precision) {
+  if (const char* f = getenv("T1M_FORCE_KERNEL")) {
+    const int k = atoi(f);
+    if (k >= 0 && k < K_COUNT) return k;
+  }
+  if (precision == T1M_PRECISION_SIMPLE) return K_SIMPLE;
+  switch (P.dtype) {
+    case T1M_COMPLEX64: {
+      const bool bf = precision == T1M_PRECISION_BF16X3;
+      const double c144 = tile_cost(P.M, P.N, 144, 144, 1, 0.90);
+      const double c64 = tile_cost(P.M, P.N, 64, 64, 2, 0.55);
+      if (bf) return c144 <= c64 ? K_C64_BF16X3_144 : K_C64_BF16X3_64;
+      return c144 <= c64 ? K_C64_F32_144 : K_C64_F32_64;
+    }
+    case T1M_COMPLEX128: {
+      const double c64 = tile_cost(P.M, P.N, 64, 64, 1, 0.80);
+      const double c32 = tile_cost(P.M, P.N, 32, 32, 2, 0.50);
+      return c64 <= c32 ? K_C128_F64_64 : K_C128_F64_32;
+    }
+    default:
+      return K_SIMPLE;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct PlanCache {
+  std::mutex mu;
+  std::list<std::string> lru;   // front = most recent
+  std::unordered_map<std::string, std::pair<std::shared_ptr<Plan>, std::list<std::string>::iterator>> map;
+  static constexpr size_t kMax = 256;
+};
+
+PlanCache& cache() {
+  static PlanCache* c = new PlanCache();   // never destroyed (no HIP at exit)
+  return *c;
+}
+
+std::string plan_key(const Problem& P, int precision, int device) {
+  std::string k;
+  char buf[64];
+  // Test / tuning overrides change the plan, so they are part of its key.
+  const char* fk = getenv("T1M_FORCE_KERNEL");
+  const char* f64 = getenv("T1M_FORCE_IDX64");
+  snprintf(buf, sizeof(buf), "%d|%d|%d|%s|%s|", device, P.dtype, precision,
+           fk ? fk : "", f64 ? f64 : "");
+  k += buf;
+  k += P.la; k += ','; k += P.lb; k += ','; k += P.lc; k += '|';
+  auto add = [&](const std::vector<int64_t>& v) {
+    for (int64_t x : v) { snprintf(buf, sizeof(buf), "%lld,", (long long)x); k += buf; }
+    k += '|';
+  };
+  add(P.da); add(P.sa); add(P.db); add(P.sb); add(P.dc); add(P.sc);
+  return k;
+}
+
+int64_t max_abs_offset(const std::vector<Mode>& modes, int which) {
+  int64_t s = 0;
+  for (const Mode& m : modes) {
+    const int64_t st = which == 0 ? m.sa : which == 1 ? m.sb : m.sc;
+    s += (m.dim > 0 ? m.dim - 1 : 0) * (st < 0 ? -st : st);
+  }
+  return s;
+}
+
+int64_t min_stride(const std::vector<Mode>& modes, int which) {
+  int64_t best = INT64_MAX;
+  for (const Mode& m : modes) {
+    if (m.dim <= 1) continue;
+    int64_t st = which == 0 ? m.sa : which == 1 ? m.sb : m.sc;
+    st = st < 0 ? -st : st;
+    best = std::min(best, st);
+  }
+  return best;
+}
+
+void order_by(std::vector<Mode>& modes, int which) {
+  std::stable_sort(modes.begin(), modes.end(), [which](const Mode& x, const Mode& y) {
+    auto key = [which](const Mode& m) {
+      if (m.dim <= 1) return INT64_MAX;
+      int64_t st = which == 0 ? m.sa : which == 1 ? m.sb : m.sc;
+      return st < 0 ? -st : st;
+    };
+    return key(x) < key(y);
+  });
+}
+
+void fill_map(MapDesc& d, const std::vector<Mode>& modes, int which, bool out32,
+              const void* out) {
+  d.nmodes = int32_t(modes.size());
+  d.out32 = out32 ? 1 : 0;
+  d.len = 1;
+  for (size_t i = 0; i < modes.size(); ++i) {
+    d.dims[i] = modes[i].dim;
+    d.strides[i] = which == 0 ? modes[i].sa : which == 1 ? modes[i].sb : modes[i].sc;
+    d.len *= modes[i].dim;
+  }
+  d.out = const_cast<void*>(out);
+}
+
+}  // namespace
+
+int get_plan(const Problem& P0, int precision, hipStream_t stream,
+             std::shared_ptr<Plan>& out) {
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
+  const std::string key = plan_key(P0, precision, device);
+  PlanCache& C = cache();
+  {
+    std::lock_guard<std::mutex> lock(C.mu);
+    auto it = C.map.find(key);
+    if (it != C.map.end()) {
+      C.lru.splice(C.lru.begin(), C.lru, it->second.second);
+      out = it->second.first;
+      return T1M_SUCCESS;
+    }
+  }
+  auto plan = std::make_shared<Plan>();
+  Problem P = P0;
+  Plan& pl = *plan;
+  pl.M = P.M; pl.N = P.N; pl.K = P.K;
+  pl.dtype = P.dtype;
+  pl.kernel = choose_kernel(P, precision);
+
+  // Staging directions and traversal order of each bundle.
+  pl.a_mfast = min_stride(P.m, 0) <= min_stride(P.k, 0) ? 1 : 0;
+  pl.b_kfast = min_stride(P.k, 1) <= min_stride(P.n, 1) ? 1 : 0;
+  order_by(P.m, pl.a_mfast ? 0 : 2);
+  order_by(P.k, pl.b_kfast ? 1 : (pl.a_mfast ? 1 : 0));
+  order_by(P.n, pl.b_kfast ? 2 : 1);
+
+  const int64_t lim = (int64_t(1) << 31) - 1;
+  const int64_t offA = max_abs_offset(P.m, 0) + max_abs_offset(P.k, 0);
+  const int64_t offB = max_abs_offset(P.k, 1) + max_abs_offset(P.n, 1);
+  const int64_t offC = max_abs_offset(P.m, 2) + max_abs_offset(P.n, 2);
+  pl.idx32 = (offA <= lim && offB <= lim && offC <= lim) ? 1 : 0;
+  if (const char* f = getenv("T1M_FORCE_IDX64"))
+    if (atoi(f) != 0) pl.idx32 = 0;
+  const size_t isz = pl.idx32 ? 4 : 8;
+
+  // One allocation: rsA, rsC (M), csA, rsB (K), csB, csC (N), each 256-B aligned.
+  auto round = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t szM = round(size_t(P.M) * isz), szK = round(size_t(P.K) * isz),
+               szN = round(size_t(P.N) * isz);
+  const size_t total = 2 * (szM + szK + szN);
+  char* base = nullptr;
+  if (hipMalloc(&base, total) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(T1M_ERROR_OUT_OF_MEMORY, "hipMalloc of %zu bytes for index maps failed", total);
+  }
+  pl.maps = base;
+  pl.rsA = base;
+  pl.rsC = base + szM;
+  pl.csA = base + 2 * szM;
+  pl.rsB = base + 2 * szM + szK;
+  pl.csB = base + 2 * szM + 2 * szK;
+  pl.csC = base + 2 * szM + 2 * szK + szN;
+
+  MapBatch mb;
+  mb.count = 6;
+  fill_map(mb.maps[0], P.m, 0, pl.idx32, pl.rsA);
+  fill_map(mb.maps[1], P.m, 2, pl.idx32, pl.rsC);
+  fill_map(mb.maps[2], P.k, 0, pl.idx32, pl.csA);
+  fill_map(mb.maps[3], P.k, 1, pl.idx32, pl.rsB);
+  fill_map(mb.maps[4], P.n, 1, pl.idx32, pl.csB);
+  fill_map(mb.maps[5], P.n, 2, pl.idx32, pl.csC);
+  hipError_t e = launch_scatter_maps(mb, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    pl.maps = nullptr;
+    return set_error(T1M_ERROR_DEVICE, "index map construction failed: %s", hipGetErrorString(e));
+  }
+  pl.modes_m = P.m; pl.modes_n = P.n; pl.modes_k = P.k;
+
+  std::lock_guard<std::mutex> lock(C.mu);
+  auto it = C.map.find(key);
+  if (it != C.map.end()) {          // another thread won the race
+    out = it->second.first;
+    return T1M_SUCCESS;
+  }
+  C.lru.push_front(key);
+  C.map.emplace(key, std::make_pair(plan, C.lru.begin()));
+  while (C.map.size() > PlanCache::kMax) {
+    C.map.erase(C.lru.back());
+    C.lru.pop_back();
+  }
+  out = plan;
+  return T1M_SUCCESS;
+}
+
+Plan::~Plan() {
+  if (maps) (void)hipFree(maps);
+}
+
+int64_t plan_cache_size() {
+  PlanCache& C = cache();
+  std::lock_guard<std::mutex> lock(C.mu);
+  return int64_t(C.map.size());
+}
+
+void plan_cache_clear() {
+  PlanCache& C = cache();
+  std::lock_guard<std::mutex> lock(C.mu);
+  C.map.clear();
+  C.lru.clear();
+}
+
+// ---------------------------------------------------------------------------
+// Host pointers: stage the strided extent through device memory.
+// ---------------------------------------------------------------------------
+
+bool is_device_pointer(const void* ptr) {
+  if (!ptr) return false;
+  hipPointerAttribute_t attr;
+  const hipError_t e = hipPointerGetAttributes(&attr, ptr);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+int Staged::stage(const void* host, const std::vector<int64_t>& dims,
+                  const std::vector<int64_t>& strides, int esize, bool copy_in,
+                  hipStream_t stream) {
+  int64_t lo = 0, hi = 0;
+  bool empty = false;
+  for (size_t i = 0; i < dims.size(); ++i) {
+    if (dims[i] == 0) empty = true;
+    const int64_t span = (dims[i] - 1) * strides[i];
+    if (span < 0) lo += span; else hi += span;
+  }
+  host_base = static_cast<const char*>(host) + lo * esize;
+  bytes = empty ? 0 : size_t(hi - lo + 1) * esize;
+  if (bytes == 0) { device = const_cast<void*>(host); return T1M_SUCCESS; }
+  if (hipMalloc(&buf, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    buf = nullptr;
+    return set_error(T1M_ERROR_OUT_OF_MEMORY, "hipMalloc of %zu bytes for staging failed", bytes);
+  }
+  device = static_cast<char*>(buf) - lo * esize;
+  if (copy_in) HIP_TRY(hipMemcpyAsync(buf, host_base, bytes, hipMemcpyHostToDevice, stream));
+  return T1M_SUCCESS;
+}
+
+int Staged::copy_back(void* host, hipStream_t stream) {
+  (void)host;
+  if (!buf || bytes == 0) return T1M_SUCCESS;
+  HIP_TRY(hipMemcpyAsync(const_cast<char*>(host_base), buf, bytes, hipMemcpyDeviceToHost, stream));
+  return T1M_SUCCESS;
+}
+
+Staged::~Staged() {
+  if (buf) (void)hipFree(buf);
+}
+
+// ---------------------------------------------------------------------------
+// Contraction driver
+// ---------------------------------------------------------------------------
+
+int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
+                  const char* lb, const t1m_tensor* c, const char* lc,
+                  const t1m_options& opt) {
+  Problem P;
+  int rc = parse_problem(a, la, b, lb, c, lc, P);
+  if (rc) return rc;
+  if (opt.precision < 0 || opt.precision > T1M_PRECISION_SIMPLE)
+    return set_error(T1M_ERROR_INVALID_ARGUMENT, "unknown precision %d", opt.precision);
+  if (opt.precision == T1M_PRECISION_BF16X3 && P.dtype != T1M_COMPLEX64)
+    return set_error(T1M_ERROR_UNSUPPORTED, "BF16x3 needs complex<float> data");
+  if (P.M == 0 || P.N == 0 || P.K == 0) return T1M_SUCCESS;   // C += 0
+  if (!a->data || !b->data || !c->data)
+    return set_error(T1M_ERROR_INVALID_ARGUMENT, "null data pointer");
+  hipStream_t stream = static_cast<hipStream_t>(opt.stream);
+
+  std::shared_ptr<Plan> plan;
+  if ((rc = get_plan(P, opt.precision, stream, plan))) return rc;
+
+  const int es = dtype_size(P.dtype);
+  Staged sa, sb, sc;
+  const void* dA = a->data;
+  const void* dB = b->data;
+  void* dC = c->data;
+  const bool hostA = !is_device_pointer(a->data);
+  const bool hostB = !is_device_pointer(b->data);
+  const bool hostC = !is_device_pointer(c->data);
+  if (hostA) { if ((rc = sa.stage(a->data, P.da, P.sa, es, true, stream))) return rc; dA = sa.device; }
+  if (hostB) { if ((rc = sb.stage(b->data, P.db, P.sb, es, true, stream))) return rc; dB = sb.device; }
+  if (hostC) { if ((rc = sc.stage(c->data, P.dc, P.sc, es, true, stream))) return rc; dC = sc.device; }
+
+  GemmProblem g;
+  g.A = dA; g.B = dB; g.C = dC;
+  g.rsA = plan->rsA; g.csA = plan->csA; g.rsB = plan->rsB; g.csB = plan->csB;
+  g.rsC = plan->rsC; g.csC = plan->csC;
+  g.M = plan->M; g.N = plan->N; g.K = plan->K;
+  g.alpha_re = opt.alpha_re; g.alpha_im = opt.alpha_im;
+  const bool cplx = P.dtype == T1M_COMPLEX64 || P.dtype == T1M_COMPLEX128;
+  g.conj_a = cplx && opt.conj_a; g.conj_b = cplx && opt.conj_b;
+  if (!cplx) g.alpha_im = 0.0;
+  g.a_mfast = plan->a_mfast; g.b_kfast = plan->b_kfast;
+  g.idx32 = plan->idx32;
+  g.dtype = P.dtype;
+  hipError_t e = launch_contract(g, plan->kernel, stream);
+  if (e != hipSuccess)
+    return set_error(T1M_ERROR_DEVICE, "kernel %s launch failed: %s",
+                     kernel_info(plan->kernel).name, hipGetErrorString(e));
+  if (hostC && (rc = sc.copy_back(c->data, stream))) return rc;
+  if (opt.synchronize || hostA || hostB || hostC) HIP_TRY(hipStreamSynchronize(stream));
+  return T1M_SUCCESS;
+}
+
+}  // namespace t1m_hip
+
+// ===========================================================================
+// extern "C"
+// ===========================================================================
+
+using namespace t1m_hip;
+
+extern "C" {
+
+void t1m_options_init(t1m_options* opt) {
+  if (!opt) return;
+  opt->alpha_re = 1.0;
+  opt->alpha_im = 0.0;
+  opt->precision = T1M_PRECISION_NATIVE;
+  opt->conj_a = 0;
+  opt->conj_b = 0;
+  opt->synchronize = 1;
+  opt->stream = nullptr;
+}
+
+const char* t1m_last_error(void) { return last_error(); }
+
+const char* t1m_version(void) { return "t1m-mi355x 0.1.0 (gfx950)"; }
+
+int64_t t1m_plan_cache_size(void) { return plan_cache_size(); }
+
+void t1m_plan_cache_clear(void) { plan_cache_clear(); }
+
+int t1m_contract(const t1m_tensor* a, const char* la, const t1m_tensor* b,
+                 const char* lb, const t1m_tensor* c, const char* lc,
+                 const t1m_options* opt) {
+  clear_error();
+  t1m_options o;
+  if (opt) o = *opt; else t1m_options_init(&o);
+  return contract_impl(a, la, b, lb, c, lc, o);
+}
+
+int t1m_contract_vjp(const t1m_tensor* a, const char* la, const t1m_tensor* b,
+                     const char* lb, const t1m_tensor* grad_c, const char* lc,
+                     const t1m_tensor* grad_a, const t1m_tensor* grad_b,
+                     const t1m_options* opt) {
+  clear_error();
+  t1m_options o;
+  if (opt) o = *opt; else t1m_options_init(&o);
+  // Validate the forward problem once so errors name A, B and C.
+  Problem P;
+  int rc = parse_problem(a, la, b, lb, grad_c, lc, P);
+  if (rc) return rc;
+  // d/dA of alpha*A.B paired with grad_c: conj(alpha) * grad_c . conj(B).
+  t1m_options g = o;
+  g.alpha_im = -o.alpha_im;
+  g.conj_a = 0;
+  if (grad_a) {
+    g.conj_b = 1;
+    if ((rc = contract_impl(grad_c, lc, b, lb, grad_a, la, g))) return rc;
+  }
+  if (grad_b) {
+    g.conj_b = 0;
+    g.conj_a = 1;
+    if ((rc = contract_impl(a, la, grad_c, lc, grad_b, lb, g))) return rc;
+  }
+  return T1M_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+"""CPU tests of the C ABI: the library loads, exports every symbol that
+include/t1m_hip.h declares, and its host-side logic (bundles, validation,
+error reporting) behaves -- no kernel is launched here."""
+import ctypes
+import os
+import random
+import re
+import string
+
+import pytest
+import torch
+
+import t1m_amd
+from oracle import oracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    with open(os.path.join(ROOT, "include", "t1m_hip.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(t1m_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    lib = t1m_amd.load()
+    names = declared_functions()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(t1m_amd.EXPORTS) == names
+
+
+def test_version():
+    assert "gfx950" in t1m_amd.version()
+
+
+def test_index_bundles_kats(kats):
+    for case in kats["bundles"]:
+        got = t1m_amd.index_bundles(case["la"], case["lb"], case["lc"])
+        for key in ("AX", "AY", "BX", "BY", "CX", "CY"):
+            assert got[key] == case[key], (case["source"], key)
+
+
+def test_index_bundles_match_oracle():
+    """Including the reference's multiset behaviour on repeated labels and
+    npos for labels missing from C."""
+    rng = random.Random(3)
+    for _ in range(300):
+        la = "".join(rng.choice("abcdefg") for _ in range(rng.randint(0, 6)))
+        lb = "".join(rng.choice("abcdefg") for _ in range(rng.randint(0, 6)))
+        lc = "".join(rng.sample(string.ascii_lowercase[:8], rng.randint(0, 8)))
+        got = t1m_amd.index_bundles(la, lb, lc)
+        want = orc.bundles(la, lb, lc)
+        for k in want:
+            want[k] = [-1 if v == 2 ** 64 - 1 else v for v in want[k]]
+        assert got == want, (la, lb, lc)
+
+
+def _t(shape, dtype=torch.complex64):
+    return torch.zeros(shape, dtype=dtype)
+
+
+@pytest.mark.parametrize("labels,shapes,status", [
+    (("aa", "bc", "ac"), ((2, 2), (3, 4), (2, 4)), 1),     # repeated label
+    (("ab", "bc", "ab"), ((2, 3), (3, 4), (2, 3)), 4),     # batch label a,b,c
+    (("ab", "cd", "ac"), ((2, 3), (3, 4), (2, 3)), 1),     # b of A nowhere
+    (("ab", "bc", "ad"), ((2, 3), (3, 4), (2, 4)), 1),     # d of C nowhere
+    (("ab", "bc", "ac"), ((2, 3), (5, 4), (2, 4)), 1),     # contracted extent
+    (("ab", "bc", "ac"), ((2, 3), (3, 4), (7, 4)), 1),     # free extent
+    (("abc", "bc", "a"), ((2, 3), (3, 4), (2,)), 1),       # rank vs labels
+])
+def test_validation_errors(labels, shapes, status):
+    a, b, c = (_t(s) for s in shapes)
+    with pytest.raises(t1m_amd.T1MError) as e:
+        t1m_amd.contract(a, labels[0], b, labels[1], c, labels[2])
+    assert e.value.status == status
+    assert t1m_amd.load().t1m_last_error().decode()
+
+
+def test_dtype_errors():
+    a, b = _t((2, 3)), _t((3, 4))
+    c = _t((2, 4), torch.complex128)
+    with pytest.raises(t1m_amd.T1MError) as e:
+        t1m_amd.contract(a, "ab", b, "bc", c, "ac")
+    assert e.value.status == 1
+    a2, b2, c2 = (_t(s, torch.complex128) for s in ((2, 3), (3, 4), (2, 4)))
+    with pytest.raises(t1m_amd.T1MError) as e:
+        t1m_amd.contract(a2, "ab", b2, "bc", c2, "ac", precision="bf16x3")
+    assert e.value.status == 4
+
+
+def test_zero_extent_is_a_no_op_without_device():
+    # K = 0: C += 0 returns before any device work.
+    a, b, c = _t((2, 0)), _t((0, 4)), _t((2, 4))
+    t1m_amd.contract(a, "ab", b, "bc", c, "ac")
+
+
+def test_bad_options_and_null_arguments():
+    lib = t1m_amd.load()
+    assert lib.t1m_contract(None, b"a", None, b"a", None, b"", None) == 1
+    assert lib.t1m_index_bundles(None, b"", b"", None, None) == 1
+    assert lib.t1m_batch_destroy(None) == 0
+    assert lib.t1m_batch_launch(None, None) == 1

@@ -1,0 +1,505 @@
+"""GPU parity tests: the HIP path through the C ABI against the CPU oracle
+(small sizes), torch complex128 / Freivalds properties (full sizes) and the
+reference's golden vectors.  Bar: index maps and packing bit-exact; values
+within the relative Frobenius tolerance written in each test."""
+import contextlib
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+import t1m_amd
+from oracle import oracle as orc
+from util import extents, rand, rel_err, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# Relative Frobenius tolerances against a long-double / complex128 reference.
+TOL = {"fp32": 2e-6, "bf16x3": 1e-5, "fp64": 1e-13, "simple32": 2e-6}
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[case] {request.node.nodeid}", flush=True)
+    yield
+    torch.cuda.synchronize()
+
+
+@contextlib.contextmanager
+def forced(kernel=None, idx64=False):
+    old_k, old_i = os.environ.get("T1M_FORCE_KERNEL"), os.environ.get("T1M_FORCE_IDX64")
+    try:
+        if kernel is not None:
+            os.environ["T1M_FORCE_KERNEL"] = str(t1m_amd.KERNELS[kernel])
+        if idx64:
+            os.environ["T1M_FORCE_IDX64"] = "1"
+        yield
+    finally:
+        for k, v in (("T1M_FORCE_KERNEL", old_k), ("T1M_FORCE_IDX64", old_i)):
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def u64(t):
+    return t.cpu().numpy().view(np.uint64)
+
+
+# ------------------------------------------------------------ index maps --
+
+MAP_CASES = [
+    # (dims, strides or None = col-major, modes, block)
+    ([3, 2, 2, 3], None, [0, 1], 3),                 # test_scatter.cpp:31-53
+    ([3, 2, 2, 3], None, [2, 3], 3),
+    ([48, 48, 48, 48], None, [0, 2], 6),             # A rows at 48^4 (a, c)
+    ([48, 48, 48, 48], None, [1, 3], 4),             # A cols (b, d), KP = 4
+    ([48, 48, 48, 48], None, [1, 0], 4),             # B rows of 'dbef' (b, d)
+    ([48, 48, 48, 48], None, [2, 3], 16),            # B cols (e, f), NR = 16
+    ([256, 4, 4, 256], None, [3, 2], 8),             # C cols of 'astc' (c, t)
+    ([7, 5, 3], [15, 1, 5], [0, 2, 1], 4),           # row-major-ish strides
+    ([5, 4, 3], [-1, 5, 40], [0, 1], 3),             # negative stride: wraps
+    ([6, 3], [0, 7], [0, 1], 5),                     # zero stride
+    ([4, 5], None, [], 2),                           # empty bundle -> {0}
+]
+
+
+@pytest.mark.parametrize("case", range(len(MAP_CASES)))
+def test_index_maps_bit_exact(case):
+    dims, strides, modes, block = MAP_CASES[case]
+    scat, bscat = t1m_amd.index_maps(dims, strides, modes, block)
+    st = strides if strides is not None else orc.col_major_strides(dims)
+    want = orc.scatter(modes, dims, [s % (1 << 64) for s in st])
+    np.testing.assert_array_equal(u64(scat), want)
+    np.testing.assert_array_equal(u64(bscat), orc.block_scatter(want, block))
+
+
+def test_index_maps_random_bit_exact():
+    rng = np.random.default_rng(17)
+    for _ in range(60):
+        nd = int(rng.integers(1, 6))
+        dims = [int(x) for x in rng.integers(1, 7, nd)]
+        strides = [int(x) for x in rng.integers(-30, 200, nd)]
+        modes = [int(x) for x in rng.permutation(nd)[:int(rng.integers(1, nd + 1))]]
+        block = int(rng.integers(1, 9))
+        scat, bscat = t1m_amd.index_maps(dims, strides, modes, block)
+        want = orc.scatter(modes, dims, [s % (1 << 64) for s in strides])
+        np.testing.assert_array_equal(u64(scat), want)
+        np.testing.assert_array_equal(u64(bscat), orc.block_scatter(want, block))
+
+
+# --------------------------------------------------- reference packing ----
+
+def _fill(kind, n, dtype):
+    if kind == "iota1":
+        return np.arange(1, n + 1, dtype=dtype)
+    i = np.arange(n)
+    return ((2 * i + 1) + 1j * (2 * i + 2)).astype(dtype)
+
+
+def _dev_layout(dims, X, Y, br, bc):
+    rs, rbs = t1m_amd.index_maps(dims, None, X, br)
+    cs, cbs = t1m_amd.index_maps(dims, None, Y, bc)
+    return rs, cs, rbs, cbs
+
+
+DT = {"float32": t1m_amd.FLOAT32, "complex64": t1m_amd.COMPLEX64}
+
+
+def test_pack_kats_on_device(kats):
+    for case in kats["pack"]:
+        dt = np.dtype(case["dtype"])
+        src = torch.from_numpy(_fill(case["fill"], case["numel"], dt)).cuda()
+        rs, cs, rbs, cbs = _dev_layout(case["dims"], case["X"], case["Y"], case["br"], case["bc"])
+        dest = torch.zeros(case["dest_len"], dtype=torch.float32, device="cuda")
+        t1m_amd.pack_block(DT[case["dtype"]], case["label"], case["br"], case["bc"],
+                           rs, cs, rbs, cbs, case["length"], src, dest)
+        assert dest.cpu().tolist() == [float(x) for x in case["expected"]], case["name"]
+
+
+def test_unpack_kats_on_device(kats):
+    for case in kats["unpack"]:
+        src = torch.arange(1, case["src_len"] + 1, dtype=torch.float32, device="cuda")
+        rs, cs, rbs, cbs = _dev_layout(case["dims"], case["X"], case["Y"], case["br"], case["bc"])
+        tdt = torch.complex64 if case["dtype"] == "complex64" else torch.float32
+        dest = torch.zeros(case["dest_numel"], dtype=tdt, device="cuda")
+        t1m_amd.unpack(DT[case["dtype"]], case["br"], case["bc"], rs, cs, rbs, cbs, src, dest)
+        flat = torch.view_as_real(dest).flatten() if dest.is_complex() else dest
+        assert flat.cpu().tolist() == [float(x) for x in case["expected"]], case["name"]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_pack_1m_random_views_bit_exact(seed):
+    """Device 1M / real panels against the oracle's pack_block on random
+    tensors, bundle splits and block sizes (odd sizes hit partial cells)."""
+    rng = np.random.default_rng(100 + seed)
+    dims = [int(x) for x in rng.integers(1, 6, 4)]
+    perm = [int(x) for x in rng.permutation(4)]
+    cut = int(rng.integers(1, 4))
+    X, Y = perm[:cut], perm[cut:]
+    n = int(np.prod(dims))
+    cplx = seed % 2 == 0
+    host = (rng.standard_normal(n) + (1j * rng.standard_normal(n) if cplx else 0)).astype(
+        np.complex64 if cplx else np.float32)
+    for label in ("A", "B"):
+        br = 2 * int(rng.integers(1, 4)) if (cplx or label == "A") else int(rng.integers(1, 6))
+        bc = 4 if label == "A" else int(rng.integers(1, 6))
+        if label == "B":
+            br, bc = 4, bc
+        lay = orc.Layout(dims, orc.col_major_strides(dims), X, Y, br, bc)
+        m, k = len(lay.rs), len(lay.cs)
+        length = (2 if cplx else 1) * (k if label == "A" else m) + int(rng.integers(0, 3))
+        npan = -(-(2 * m if cplx else m) // br) if label == "A" else -(-k // bc)
+        size = npan * (br if label == "A" else bc) * length + 64
+        want = np.zeros(size, dtype=np.float32)
+        orc.pack_block(lay, label, length, host, want)
+        rs, cs, rbs, cbs = _dev_layout(dims, X, Y, br, bc)
+        got = torch.zeros(size, dtype=torch.float32, device="cuda")
+        t1m_amd.pack_block(t1m_amd.COMPLEX64 if cplx else t1m_amd.FLOAT32, label, br, bc,
+                           rs, cs, rbs, cbs, length, torch.from_numpy(host).cuda(), got)
+        np.testing.assert_array_equal(got.cpu().numpy(), want)
+
+
+# ----------------------------------------------------------- contraction --
+
+def run(a_np, la, b_np, lb, c_np, lc, **kw):
+    a, b, c = to_dev(a_np), to_dev(b_np), to_dev(c_np)
+    t1m_amd.contract(a, la, b, lb, c, lc, **kw)
+    return to_host(c)
+
+
+def oracle_einsum(a, la, b, lb, c0, lc, alpha=1.0, conj_a=False, conj_b=False):
+    c = np.asfortranarray(c0.copy())
+    orc.einsum(a, la, b, lb, c, lc, alpha=alpha, conj_a=conj_a, conj_b=conj_b)
+    return c
+
+
+def test_readme_example_c1_all_paths():
+    """A[10,10,10] 'abc' x B[10,10] 'bd' -> C[10,10,10] 'acd' (example.cpp)."""
+    rng = np.random.default_rng(1)
+    A, B = rand(rng, (10, 10, 10), np.complex64), rand(rng, (10, 10), np.complex64)
+    C0 = np.zeros((10, 10, 10), np.complex64, order="F")
+    want = C0.copy(order="F")
+    orc.contract(A, "abc", B, "bd", want, "acd")      # the reference's 5 loops
+    exact = oracle_einsum(A, "abc", B, "bd", C0, "acd")
+    assert rel_err(want, exact) < 1e-6
+    for kern, prec, tol in [("c64_f32mfma_144x144", "native", TOL["fp32"]),
+                            ("c64_f32mfma_64x64", "native", TOL["fp32"]),
+                            ("c64_bf16x3_144x144", "bf16x3", TOL["bf16x3"]),
+                            ("c64_bf16x3_64x64", "bf16x3", TOL["bf16x3"]),
+                            ("simple", "native", TOL["simple32"])]:
+        with forced(kern):
+            got = run(A, "abc", B, "bd", C0, "acd", precision=prec)
+        assert rel_err(got, exact) < tol, kern
+        assert rel_err(got, want) < 2 * tol, kern
+
+
+RAGGED = [
+    # labels, extents (M, N, K all ragged w.r.t. 144 / 64 / 32 tiles)
+    (("abcd", "dbef", "acef"), {"a": 7, "b": 5, "c": 23, "d": 13, "e": 9, "f": 17}),
+    (("asb", "btc", "astc"), {"a": 37, "s": 4, "b": 19, "t": 4, "c": 29}),
+    (("efbad", "cf", "abcde"), {"a": 6, "b": 5, "c": 11, "d": 4, "e": 7, "f": 9}),
+    (("ab", "bc", "ac"), {"a": 300, "b": 3, "c": 290}),
+    (("ba", "cb", "ca"), {"a": 150, "b": 41, "c": 70}),
+]
+
+KERNELS_C64 = [("c64_f32mfma_144x144", "native", "fp32"), ("c64_f32mfma_64x64", "native", "fp32"),
+               ("c64_bf16x3_144x144", "bf16x3", "bf16x3"), ("c64_bf16x3_64x64", "bf16x3", "bf16x3"),
+               ("simple", "native", "simple32")]
+KERNELS_C128 = [("c128_f64mfma_64x64", "native", "fp64"), ("c128_f64mfma_32x32", "native", "fp64"),
+                ("simple", "native", "fp64")]
+
+
+@pytest.mark.parametrize("idx64", [False, True])
+@pytest.mark.parametrize("case", range(len(RAGGED)))
+def test_ragged_all_kernels(case, idx64):
+    (la, lb, lc), ext = RAGGED[case]
+    rng = np.random.default_rng(case)
+    for dtype, kernels in ((np.complex64, KERNELS_C64), (np.complex128, KERNELS_C128)):
+        A = rand(rng, [ext[x] for x in la], dtype)
+        B = rand(rng, [ext[x] for x in lb], dtype)
+        C0 = rand(rng, [ext[x] for x in lc], dtype)
+        want = oracle_einsum(A, la, B, lb, C0, lc)
+        for kern, prec, tolkey in kernels:
+            with forced(kern, idx64):
+                got = run(A, la, B, lb, C0, lc, precision=prec)
+            err = rel_err(got - C0, want - C0)
+            print(f"  {la},{lb}->{lc} {np.dtype(dtype).name} {kern} idx64={idx64} rel={err:.2e}")
+            assert err < TOL[tolkey], (kern, err)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_real_contractions(dtype):
+    """Real contract(alpha, ...) (t1m.h:560-568): C += alpha * A . B."""
+    rng = np.random.default_rng(5)
+    for (la, lb, lc), ext in RAGGED[:3]:
+        A = rand(rng, [ext[x] for x in la], dtype)
+        B = rand(rng, [ext[x] for x in lb], dtype)
+        C0 = rand(rng, [ext[x] for x in lc], dtype)
+        want = oracle_einsum(A, la, B, lb, C0, lc, alpha=0.75)
+        got = run(A, la, B, lb, C0, lc, alpha=0.75)
+        assert rel_err(got - C0, want - C0) < (2e-6 if dtype == np.float32 else 1e-13)
+
+
+def test_conj_alpha_and_accumulate():
+    rng = np.random.default_rng(8)
+    la, lb, lc = "abcd", "dbef", "acef"
+    ext = {"a": 9, "b": 6, "c": 17, "d": 11, "e": 8, "f": 19}
+    for dtype, kernels in ((np.complex64, KERNELS_C64), (np.complex128, KERNELS_C128)):
+        A = rand(rng, [ext[x] for x in la], dtype)
+        B = rand(rng, [ext[x] for x in lb], dtype)
+        C0 = rand(rng, [ext[x] for x in lc], dtype)
+        for kern, prec, tolkey in kernels:
+            for ca, cb, alpha in [(True, False, 1.0), (False, True, 0.5 - 2j), (True, True, -1j)]:
+                want = oracle_einsum(A, la, B, lb, C0, lc, alpha=alpha, conj_a=ca, conj_b=cb)
+                with forced(kern):
+                    got = run(A, la, B, lb, C0, lc, alpha=alpha, conj_a=ca, conj_b=cb,
+                              precision=prec)
+                assert rel_err(got - C0, want - C0) < TOL[tolkey], (kern, ca, cb, alpha)
+        # C += semantics: a second call adds the product again
+        a, b, c = to_dev(A), to_dev(B), to_dev(C0)
+        t1m_amd.contract(a, la, b, lb, c, lc)
+        t1m_amd.contract(a, la, b, lb, c, lc)
+        twice = oracle_einsum(A, la, B, lb, oracle_einsum(A, la, B, lb, C0, lc), lc)
+        tol = TOL["fp32"] if dtype == np.complex64 else TOL["fp64"]
+        assert rel_err(to_host(c) - C0, twice - C0) < tol
+
+
+def test_edge_shapes():
+    rng = np.random.default_rng(9)
+    cases = [
+        ("a", "b", "ab", {"a": 70, "b": 33}),                 # outer product (K = 1)
+        ("ab", "ab", "", {"a": 20, "b": 31}),                 # full contraction -> scalar
+        ("abc", "c", "ab", {"a": 5, "b": 9, "c": 300}),       # N = 1 (matrix-vector)
+        ("c", "cb", "b", {"b": 150, "c": 77}),                # M = 1
+        ("ab", "bc", "ac", {"a": 1, "b": 1, "c": 1}),         # 1 x 1 x 1
+        ("abc", "cbd", "da", {"a": 17, "b": 3, "c": 2, "d": 145}),
+    ]
+    for la, lb, lc, ext in cases:
+        for dtype in (np.complex64, np.complex128):
+            A = rand(rng, [ext[x] for x in la], dtype)
+            B = rand(rng, [ext[x] for x in lb], dtype)
+            C0 = rand(rng, [ext[x] for x in lc], dtype)
+            want = oracle_einsum(A, la, B, lb, C0, lc)
+            got = run(A, la, B, lb, C0, lc)
+            tol = TOL["fp32"] if dtype == np.complex64 else TOL["fp64"]
+            assert rel_err(got - C0, want - C0) < tol, (la, lb, lc, dtype)
+
+
+def test_row_major_and_negative_strides():
+    rng = np.random.default_rng(10)
+    la, lb, lc = "abc", "cbd", "ad"
+    ext = {"a": 33, "b": 5, "c": 21, "d": 40}
+    A = np.ascontiguousarray(rand(rng, [ext[x] for x in la], np.complex64))   # row-major
+    B = rand(rng, [ext[x] for x in lb], np.complex64)
+    C0 = np.ascontiguousarray(rand(rng, [ext[x] for x in lc], np.complex64))
+    want = oracle_einsum(A, la, B, lb, C0, lc)
+    a, b = torch.from_numpy(A).cuda(), to_dev(B)
+    c = torch.from_numpy(C0).cuda()
+    t1m_amd.contract(a, la, b, lb, c, lc)
+    assert rel_err(c.cpu().numpy() - C0, want - C0) < TOL["fp32"]
+    # A viewed backwards along 'a' through a descriptor with a negative stride
+    Af = A[::-1].copy()                       # Af[i] = A[n-1-i]
+    af = torch.from_numpy(np.ascontiguousarray(Af)).cuda()
+    last = af.data_ptr() + (ext["a"] - 1) * af.stride(0) * af.element_size()
+    d = t1m_amd._Desc(af, strides=[-af.stride(0), af.stride(1), af.stride(2)], ptr=last)
+    c2 = torch.from_numpy(C0).cuda()
+    db, dc = t1m_amd._Desc(b), t1m_amd._Desc(c2)
+    o = t1m_amd._options()
+    rc = t1m_amd.load().t1m_contract(ctypes.byref(d.desc), la.encode(), ctypes.byref(db.desc),
+                                      lb.encode(), ctypes.byref(dc.desc), lc.encode(),
+                                      ctypes.byref(o))
+    assert rc == 0
+    assert rel_err(c2.cpu().numpy() - C0, want - C0) < TOL["fp32"]
+
+
+def test_host_pointers_are_staged():
+    """Host memory, as the reference's callers pass it, through the C ABI."""
+    rng = np.random.default_rng(11)
+    A = rand(rng, (12, 7, 9), np.complex64)
+    B = rand(rng, (7, 11), np.complex64)
+    C = rand(rng, (12, 9, 11), np.complex64)
+    C0 = C.copy(order="F")
+    want = oracle_einsum(A, "abc", B, "bd", C0, "acd")
+
+    def desc(x):
+        dims = (ctypes.c_int64 * x.ndim)(*x.shape)
+        st = (ctypes.c_int64 * x.ndim)(*[s // x.itemsize for s in x.strides])
+        return t1m_amd.TensorDesc(t1m_amd.COMPLEX64, x.ndim, dims, st, x.ctypes.data), dims, st
+
+    da, db, dc = desc(A), desc(B), desc(C)
+    o = t1m_amd._options(stream=0)
+    rc = t1m_amd.load().t1m_contract(ctypes.byref(da[0]), b"abc", ctypes.byref(db[0]), b"bd",
+                                      ctypes.byref(dc[0]), b"acd", ctypes.byref(o))
+    assert rc == 0, t1m_amd.load().t1m_last_error()
+    assert rel_err(C - C0, want - C0) < TOL["fp32"]
+
+
+def test_plan_cache_reuse():
+    rng = np.random.default_rng(12)
+    A, B = rand(rng, (8, 9), np.complex64), rand(rng, (9, 10), np.complex64)
+    a, b = to_dev(A), to_dev(B)
+    c = torch.zeros(8, 10, dtype=torch.complex64, device="cuda")
+    t1m_amd.contract(a, "ab", b, "bc", c, "ac")
+    n = t1m_amd.plan_cache_size()
+    for _ in range(5):
+        t1m_amd.contract(a, "ab", b, "bc", c, "ac")
+    assert t1m_amd.plan_cache_size() == n
+
+
+# ----------------------------------------------------- full-size configs --
+
+def _freivalds(A, la, B, lb, C_delta, lc, rng):
+    """Check C_delta = A.B with a random vector over C's B-free labels:
+    (A.B).x == A.(B.x), all in complex128 numpy (size independent)."""
+    fb = [ch for ch in lb if ch not in la]
+    x = rng.standard_normal([B.shape[lb.index(ch)] for ch in fb]) + 0j
+    fbl = "".join(fb)
+    lhs = np.einsum(f"{lc},{fbl}->{''.join(ch for ch in lc if ch not in fbl)}", C_delta, x,
+                    optimize=True)
+    bx = np.einsum(f"{lb},{fbl}->{''.join(ch for ch in lb if ch not in fbl)}", B, x,
+                   optimize=True)
+    rhs = np.einsum(f"{la},{''.join(ch for ch in lb if ch not in fbl)}->"
+                    f"{''.join(ch for ch in lc if ch not in fbl)}", A, bx, optimize=True)
+    return rel_err(lhs, rhs)
+
+
+def _torch_ref(a, la, b, lb, lc):
+    return torch.einsum(f"{la},{lb}->{lc}", a.to(torch.complex128), b.to(torch.complex128))
+
+
+def test_config2_48x4_fp32_full_size():
+    """c2: complex<float> A[48^4] 'abcd' x B[48^4] 'dbef' -> C[48^4] 'acef'."""
+    rng = np.random.default_rng(2)
+    sh = (48, 48, 48, 48)
+    A, B = rand(rng, sh, np.complex64), rand(rng, sh, np.complex64)
+    a, b = to_dev(A), to_dev(B)
+    c = to_dev(np.zeros(sh, np.complex64, order="F"))
+    t1m_amd.contract(a, "abcd", b, "dbef", c, "acef")
+    ref = _torch_ref(a, "abcd", b, "dbef", "acef")
+    err = torch.linalg.vector_norm(c.to(torch.complex128) - ref) / torch.linalg.vector_norm(ref)
+    print(f"  c2 fp32 rel_err vs complex128 = {err.item():.3e}")
+    assert err.item() < TOL["fp32"]
+    C = to_host(c)
+    f = _freivalds(A.astype(np.complex128), "abcd", B.astype(np.complex128), "dbef", C, "acef", rng)
+    assert f < TOL["fp32"]
+
+
+def test_config3_tn_step_fp64_full_size():
+    """c3: complex<double> A[256,4,256] 'asb' x B[256,4,256] 'btc' -> C 'astc'."""
+    rng = np.random.default_rng(3)
+    A = rand(rng, (256, 4, 256), np.complex128, "normal") / 16.0
+    B = rand(rng, (256, 4, 256), np.complex128, "normal") / 16.0
+    a, b = to_dev(np.asfortranarray(A)), to_dev(np.asfortranarray(B))
+    c = to_dev(np.zeros((256, 4, 4, 256), np.complex128, order="F"))
+    t1m_amd.contract(a, "asb", b, "btc", c, "astc")
+    ref = _torch_ref(a, "asb", b, "btc", "astc")
+    err = (torch.linalg.vector_norm(c - ref) / torch.linalg.vector_norm(ref)).item()
+    print(f"  c3 fp64 rel_err vs complex128 = {err:.3e}")
+    assert err < TOL["fp64"]
+
+
+def test_config4_precision_bf16x3_vs_fp32():
+    """c4: c2 shapes, |z| log-uniform in [1e-3, 1e3], random phase."""
+    rng = np.random.default_rng(4)
+    sh = (48, 48, 48, 48)
+    A, B = rand(rng, sh, np.complex64, "logmag"), rand(rng, sh, np.complex64, "logmag")
+    a, b = to_dev(A), to_dev(B)
+    ref = _torch_ref(a, "abcd", b, "dbef", "acef")
+    errs = {}
+    for prec in ("native", "bf16x3"):
+        c = to_dev(np.zeros(sh, np.complex64, order="F"))
+        t1m_amd.contract(a, "abcd", b, "dbef", c, "acef", precision=prec)
+        errs[prec] = (torch.linalg.vector_norm(c.to(torch.complex128) - ref) /
+                      torch.linalg.vector_norm(ref)).item()
+    print(f"  c4 rel_err fp32={errs['native']:.3e} bf16x3={errs['bf16x3']:.3e}")
+    assert errs["native"] < TOL["fp32"]
+    assert errs["bf16x3"] < TOL["bf16x3"]
+
+
+# ------------------------------------------------------------------ VJP --
+
+def test_vjp_against_oracle_and_finite_differences():
+    """VJP of L = sum |C|^2 (grad_c = 2C): grad_a = grad_c . conj(B),
+    grad_b = conj(A) . grad_c, checked against the oracle's conjugated label
+    loops and complex128 central differences (h = 1e-6)."""
+    rng = np.random.default_rng(13)
+    la, lb, lc = "asb", "btc", "astc"
+    ext = {"a": 6, "s": 4, "b": 5, "t": 4, "c": 7}
+    A = rand(rng, [ext[x] for x in la], np.complex128)
+    B = rand(rng, [ext[x] for x in lb], np.complex128)
+    Cf = oracle_einsum(A, la, B, lb, np.zeros([ext[x] for x in lc], np.complex128, order="F"), lc)
+    G = 2 * Cf
+    ga_want = oracle_einsum(G, lc, B, lb, np.zeros_like(A, order="F"), la, conj_b=True)
+    gb_want = oracle_einsum(A, la, G, lc, np.zeros_like(B, order="F"), lb, conj_a=True)
+    a, b, g = to_dev(A), to_dev(B), to_dev(np.asfortranarray(G))
+    ga = to_dev(np.zeros_like(A, order="F"))
+    gb = to_dev(np.zeros_like(B, order="F"))
+    t1m_amd.contract_vjp(a, la, b, lb, g, lc, ga, gb)
+    assert rel_err(to_host(ga), ga_want) < 1e-13
+    assert rel_err(to_host(gb), gb_want) < 1e-13
+
+    def loss(Ax, Bx):
+        return float(np.sum(np.abs(np.einsum(f"{la},{lb}->{lc}", Ax, Bx)) ** 2))
+
+    h = 1e-6
+    GA = to_host(ga)
+    for _ in range(6):
+        idx = tuple(int(rng.integers(0, s)) for s in A.shape)
+        for unit, part in ((1.0, "re"), (1j, "im")):
+            Ap, Am = A.copy(), A.copy()
+            Ap[idx] += unit * h
+            Am[idx] -= unit * h
+            fd = (loss(Ap, B) - loss(Am, B)) / (2 * h)
+            an = GA[idx].real if part == "re" else GA[idx].imag
+            assert abs(fd - an) <= 1e-5 * max(1.0, abs(fd)), (idx, part, fd, an)
+
+
+def test_autograd_harness_complex64():
+    rng = np.random.default_rng(14)
+    A = to_dev(rand(rng, (9, 4, 7), np.complex64)).requires_grad_(True)
+    B = to_dev(rand(rng, (7, 4, 8), np.complex64)).requires_grad_(True)
+    C = t1m_amd.contract_autograd(A, "asb", B, "btc", "astc")
+    (C.abs() ** 2).sum().backward()
+    A2 = A.detach().to(torch.complex128).requires_grad_(True)
+    B2 = B.detach().to(torch.complex128).requires_grad_(True)
+    (torch.einsum("asb,btc->astc", A2, B2).abs() ** 2).sum().backward()
+    assert rel_err(A.grad.cpu().numpy(), A2.grad.cpu().numpy()) < 1e-5
+    assert rel_err(B.grad.cpu().numpy(), B2.grad.cpu().numpy()) < 1e-5
+
+
+# ---------------------------------------------------------------- batch --
+
+def test_batch_matches_single_calls():
+    rng = np.random.default_rng(15)
+    la, lb, lc = "asb", "btc", "astc"
+    As, Bs, Cs, wants = [], [], [], []
+    for _ in range(12):
+        dl, db, dr = (int(x) for x in rng.choice([16, 32, 48, 64, 96, 128], 3))
+        A = rand(rng, (dl, 4, db), np.complex64)
+        B = rand(rng, (db, 4, dr), np.complex64)
+        C0 = rand(rng, (dl, 4, 4, dr), np.complex64)
+        wants.append(oracle_einsum(A, la, B, lb, C0, lc) - C0)
+        As.append(to_dev(A)); Bs.append(to_dev(B)); Cs.append(to_dev(C0))
+        Cs[-1]._c0 = C0
+    batch = t1m_amd.Batch(As, la, Bs, lb, Cs, lc)
+    for rep in range(3):
+        batch.launch()
+        torch.cuda.synchronize()
+        for c, w in zip(Cs, wants):
+            assert rel_err(to_host(c) - c._c0, (rep + 1) * w) < TOL["fp32"]
+    batch.close()
+
+
+# ------------------------------------------------------- C++ drop-in ----
+
+def test_cpp_dropin_program():
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "dropin_test")
+    assert os.path.exists(exe), "build tests/cpp first (__graft_entry__.build())"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-4000:])
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]

@@ -61,8 +61,6 @@ k_contract_bf16x3(KArgs<Idx> p) {
   constexpr bool XT = (NBT % 2) != 0;
   constexpr int XB = XT ? (MB + 1) / 2 : 0;
   constexpr int XB_ALLOC = XT ? XB : 1;
-  constexpr int EA = BM * BK / 256;
-  constexpr int EB = BK * BN / 256;
   constexpr int PITCH = BK + 4;          // words per LDS row
   static_assert(BK % 16 == 0, "16 complex k per MFMA");
   static_assert(BM % 16 == 0 && BN % 16 == 0, "tile shape");
@@ -83,75 +81,34 @@ k_contract_bf16x3(KArgs<Idx> p) {
   const float2* __restrict__ A = static_cast<const float2*>(p.A);
   const float2* __restrict__ B = static_cast<const float2*>(p.B);
 
-  Idx aRow[EA];
-  uint32_t aOk = 0;
-#pragma unroll
-  for (int u = 0; u < EA; ++u) {
-    const int e = tid + 256 * u;
-    const int m = p.a_mfast ? e % BM : e / BK;
-    const int64_t ig = i0 + m;
-    aRow[u] = ig < M ? p.rsA[ig] : Idx(0);
-    aOk |= (ig < M ? 1u : 0u) << u;
-  }
-  Idx bCol[EB];
-  uint32_t bOk = 0;
-#pragma unroll
-  for (int u = 0; u < EB; ++u) {
-    const int e = tid + 256 * u;
-    const int n = p.b_kfast ? e / BK : e % BN;
-    const int64_t jg = j0 + n;
-    bCol[u] = jg < N ? p.csB[jg] : Idx(0);
-    bOk |= (jg < N ? 1u : 0u) << u;
-  }
-
-  float2 ra[EA], rb[EB];
+  const unsigned utid = unsigned(tid);
+  const bool afast = p.a_mfast != 0;   // A lanes along m
+  const bool bfast = p.b_kfast == 0;   // B lanes along n
+  TileGather<float2, Idx, BM, BK> ga;
+  TileGather<float2, Idx, BN, BK> gb;
+  ga.init(utid, i0, M, p.rsA, afast);
+  gb.init(utid, j0, N, p.csB, bfast);
   const float sa = p.conj_a ? -1.f : 1.f;
   const float sb = p.conj_b ? -1.f : 1.f;
 
-  auto gload = [&](int64_t p0) {
+  auto swrite = [&](int buf, int64_t p0) {
+    ga.finish(utid, p0, K, afast, sa);
+    gb.finish(utid, p0, K, bfast, sb);
 #pragma unroll
-    for (int u = 0; u < EA; ++u) {
-      const int e = tid + 256 * u;
-      const int k = p.a_mfast ? e / BM : e % BK;
-      const int64_t kg = p0 + k;
-      float2 v = make_float2(0.f, 0.f);
-      if (((aOk >> u) & 1u) && kg < K) {
-        v = A[aRow[u] + p.csA[kg]];
-        v.y *= sa;
-      }
-      ra[u] = v;
-    }
-#pragma unroll
-    for (int u = 0; u < EB; ++u) {
-      const int e = tid + 256 * u;
-      const int k = p.b_kfast ? e % BK : e / BN;
-      const int64_t kg = p0 + k;
-      float2 v = make_float2(0.f, 0.f);
-      if (((bOk >> u) & 1u) && kg < K) {
-        v = B[p.rsB[kg] + bCol[u]];
-        v.y *= sb;
-      }
-      rb[u] = v;
-    }
-  };
-  auto swrite = [&](int buf) {
-#pragma unroll
-    for (int u = 0; u < EA; ++u) {
-      const int e = tid + 256 * u;
-      const int m = p.a_mfast ? e % BM : e / BK;
-      const int k = p.a_mfast ? e / BM : e % BK;
+    for (int u = 0; u < ga.E; ++u) {
+      const unsigned e = utid + 256u * u;
       uint32_t hi, lo;
-      split_pair(ra[u].x, ra[u].y, hi, lo);
+      split_pair(ga.val[u].x, ga.val[u].y, hi, lo);
+      const int m = ga.xc(e, afast), k = ga.kc(e, afast);
       sAh[buf][m][k] = hi;
       sAl[buf][m][k] = lo;
     }
 #pragma unroll
-    for (int u = 0; u < EB; ++u) {
-      const int e = tid + 256 * u;
-      const int n = p.b_kfast ? e / BK : e % BN;
-      const int k = p.b_kfast ? e % BK : e / BN;
+    for (int u = 0; u < gb.E; ++u) {
+      const unsigned e = utid + 256u * u;
       uint32_t hi, lo;
-      split_pair(rb[u].x, rb[u].y, hi, lo);
+      split_pair(gb.val[u].x, gb.val[u].y, hi, lo);
+      const int n = gb.xc(e, bfast), k = gb.kc(e, bfast);
       sBh[buf][n][k] = hi;
       sBl[buf][n][k] = lo;
     }
@@ -229,14 +186,28 @@ k_contract_bf16x3(KArgs<Idx> p) {
   };
 
   const int nst = int((K + BK - 1) / BK);
-  gload(0);
-  swrite(0);
+  ga.load_k(utid, 0, K, p.csA, afast);
+  gb.load_k(utid, 0, K, p.rsB, bfast);
+  ga.load_data(A);
+  gb.load_data(B);
+  if (nst > 1) {
+    ga.load_k(utid, BK, K, p.csA, afast);
+    gb.load_k(utid, BK, K, p.rsB, bfast);
+  }
+  swrite(0, 0);
   __syncthreads();
   for (int s = 0; s < nst; ++s) {
     const bool more = s + 1 < nst;
-    if (more) gload(int64_t(s + 1) * BK);
+    if (more) {
+      ga.load_data(A);
+      gb.load_data(B);
+    }
+    if (s + 2 < nst) {
+      ga.load_k(utid, int64_t(s + 2) * BK, K, p.csA, afast);
+      gb.load_k(utid, int64_t(s + 2) * BK, K, p.rsB, bfast);
+    }
     compute(s & 1);
-    if (more) swrite((s + 1) & 1);
+    if (more) swrite((s + 1) & 1, int64_t(s + 1) * BK);
     __syncthreads();
   }
 

@@ -49,8 +49,6 @@ k_contract_1m(KArgs<Idx> p) {
   constexpr bool XT = (NBT % 2) != 0;    // shared middle column block
   constexpr int XB = XT ? (MB + 1) / 2 : 0;
   constexpr int XB_ALLOC = XT ? XB : 1;
-  constexpr int EA = BM * BK / 256;      // A elements staged per lane
-  constexpr int EB = BK * BN / 256;
   // Row pitch padding (complex elements) so the four lane groups of a
   // fragment read hit different LDS banks.
   constexpr int PADA = sizeof(T) == 4 ? 8 : 4;
@@ -74,74 +72,29 @@ k_contract_1m(KArgs<Idx> p) {
   const C2* __restrict__ A = static_cast<const C2*>(p.A);
   const C2* __restrict__ B = static_cast<const C2*>(p.B);
 
-  // ---- per-lane staging coordinates (fixed for the whole K loop) --------
-  Idx aRow[EA];
-  uint32_t aOk = 0;
-#pragma unroll
-  for (int u = 0; u < EA; ++u) {
-    const int e = tid + 256 * u;
-    const int m = p.a_mfast ? e % BM : e / BK;
-    const int64_t ig = i0 + m;
-    aRow[u] = ig < M ? p.rsA[ig] : Idx(0);
-    aOk |= (ig < M ? 1u : 0u) << u;
-  }
-  Idx bCol[EB];
-  uint32_t bOk = 0;
-#pragma unroll
-  for (int u = 0; u < EB; ++u) {
-    const int e = tid + 256 * u;
-    const int n = p.b_kfast ? e / BK : e % BN;
-    const int64_t jg = j0 + n;
-    bCol[u] = jg < N ? p.csB[jg] : Idx(0);
-    bOk |= (jg < N ? 1u : 0u) << u;
-  }
-
-  C2 ra[EA], rb[EB];
+  // ---- gathers: A tile (BM x BK), B tile (BN x BK) -----------------------
+  const unsigned utid = unsigned(tid);
+  const bool afast = p.a_mfast != 0;   // A lanes along m
+  const bool bfast = p.b_kfast == 0;   // B lanes along n
+  TileGather<C2, Idx, BM, BK> ga;
+  TileGather<C2, Idx, BN, BK> gb;
+  ga.init(utid, i0, M, p.rsA, afast);
+  gb.init(utid, j0, N, p.csB, bfast);
   const T sa = p.conj_a ? T(-1) : T(1);
   const T sb = p.conj_b ? T(-1) : T(1);
 
-  auto gload = [&](int64_t p0) {
+  auto swrite = [&](int buf, int64_t p0) {
+    ga.finish(utid, p0, K, afast, sa);
+    gb.finish(utid, p0, K, bfast, sb);
 #pragma unroll
-    for (int u = 0; u < EA; ++u) {
-      const int e = tid + 256 * u;
-      const int k = p.a_mfast ? e / BM : e % BK;
-      const int64_t kg = p0 + k;
-      C2 v;
-      v.x = T(0); v.y = T(0);
-      if (((aOk >> u) & 1u) && kg < K) {
-        v = A[aRow[u] + p.csA[kg]];
-        v.y *= sa;
-      }
-      ra[u] = v;
+    for (int u = 0; u < ga.E; ++u) {
+      const unsigned e = utid + 256u * u;
+      sA[buf][ga.kc(e, afast)][ga.xc(e, afast)] = ga.val[u];
     }
 #pragma unroll
-    for (int u = 0; u < EB; ++u) {
-      const int e = tid + 256 * u;
-      const int k = p.b_kfast ? e % BK : e / BN;
-      const int64_t kg = p0 + k;
-      C2 v;
-      v.x = T(0); v.y = T(0);
-      if (((bOk >> u) & 1u) && kg < K) {
-        v = B[p.rsB[kg] + bCol[u]];
-        v.y *= sb;
-      }
-      rb[u] = v;
-    }
-  };
-  auto swrite = [&](int buf) {
-#pragma unroll
-    for (int u = 0; u < EA; ++u) {
-      const int e = tid + 256 * u;
-      const int m = p.a_mfast ? e % BM : e / BK;
-      const int k = p.a_mfast ? e / BM : e % BK;
-      sA[buf][k][m] = ra[u];
-    }
-#pragma unroll
-    for (int u = 0; u < EB; ++u) {
-      const int e = tid + 256 * u;
-      const int n = p.b_kfast ? e / BK : e % BN;
-      const int k = p.b_kfast ? e % BK : e / BN;
-      sB[buf][k][n] = rb[u];
+    for (int u = 0; u < gb.E; ++u) {
+      const unsigned e = utid + 256u * u;
+      sB[buf][gb.kc(e, bfast)][gb.xc(e, bfast)] = gb.val[u];
     }
   };
 
@@ -204,15 +157,31 @@ k_contract_1m(KArgs<Idx> p) {
   };
 
   // ---- main loop: double-buffered LDS, one barrier per stage ------------
+  // Iteration s multiplies stage s from LDS while the data loads of stage
+  // s+1 and the k-offset loads of stage s+2 are in flight.
   const int nst = int((K + BK - 1) / BK);
-  gload(0);
-  swrite(0);
+  ga.load_k(utid, 0, K, p.csA, afast);
+  gb.load_k(utid, 0, K, p.rsB, bfast);
+  ga.load_data(A);
+  gb.load_data(B);
+  if (nst > 1) {
+    ga.load_k(utid, BK, K, p.csA, afast);
+    gb.load_k(utid, BK, K, p.rsB, bfast);
+  }
+  swrite(0, 0);
   __syncthreads();
   for (int s = 0; s < nst; ++s) {
     const bool more = s + 1 < nst;
-    if (more) gload(int64_t(s + 1) * BK);
+    if (more) {
+      ga.load_data(A);
+      gb.load_data(B);
+    }
+    if (s + 2 < nst) {
+      ga.load_k(utid, int64_t(s + 2) * BK, K, p.csA, afast);
+      gb.load_k(utid, int64_t(s + 2) * BK, K, p.rsB, bfast);
+    }
     compute(s & 1);
-    if (more) swrite((s + 1) & 1);
+    if (more) swrite((s + 1) & 1, int64_t(s + 1) * BK);
     __syncthreads();
   }
 
@@ -232,7 +201,9 @@ k_contract_1m(KArgs<Idx> p) {
       } else {
         val = ar * val;
       }
-      const int64_t j = jb + r;
+      // MFMA output row of (lane, r): 4(lane/16) + r for the f32 16x16x4
+      // form, lane/16 + 4r for the f64 one.
+      const int64_t j = jb + (sizeof(T) == 4 ? 4 * g + r : g + 4 * r);
       if (iok && j < N) {
         T* dst = Cr + 2 * int64_t(rc + p.csC[j]) + (odd ? 1 : 0);
         *dst += val;
@@ -244,14 +215,14 @@ k_contract_1m(KArgs<Idx> p) {
     const int64_t i = i0 + am0 + mb * 8;
 #pragma unroll
     for (int nb = 0; nb < NBW; ++nb)
-      store(acc[mb][nb], i, j0 + wn * (NBW + (XT ? 1 : 0)) * 16 + nb * 16 + g * 4);
+      store(acc[mb][nb], i, j0 + wn * (NBW + (XT ? 1 : 0)) * 16 + nb * 16);
   }
   if constexpr (XT) {
 #pragma unroll
     for (int x = 0; x < XB; ++x) {
       const int mb = wn == 0 ? x : XB + x;
       if (mb < MB)
-        store(accx[x], i0 + am0 + mb * 8, j0 + NBW * 16 + g * 4);
+        store(accx[x], i0 + am0 + mb * 8, j0 + NBW * 16);
     }
   }
 }

@@ -58,6 +58,68 @@ __device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
 }
 
 
+// Gather of one operand tile (BX free-index values x BK contracted values)
+// from strided global memory, for 256 lanes.  Element e = tid + 256 u of the
+// tile sits at (x, k) = (e % BX, e / BX) when lanes run along the free index
+// (xfast) and at (e / BK, e % BK) otherwise; the choice follows the
+// operand's unit-stride direction, so consecutive lanes read consecutive
+// addresses.  Loads are issued without branches: out-of-range rows / k read
+// a valid in-range element and are zeroed when the tile is written to LDS,
+// after the main loop's multiply has overlapped their latency.  The k
+// offsets of the next stage are fetched one stage ahead (load_k) so no data
+// load waits on an index load.
+template <typename T2, typename Idx, int BX, int BK>
+struct TileGather {
+  static constexpr int E = BX * BK / 256;
+  static_assert((BX * BK) % 256 == 0, "tile must split evenly over 256 lanes");
+  Idx xoff[E];
+  Idx koff[E];
+  T2 val[E];
+  uint32_t xok;
+
+  __device__ static int xc(unsigned e, bool xfast) {
+    return int(xfast ? e % unsigned(BX) : e / unsigned(BK));
+  }
+  __device__ static int kc(unsigned e, bool xfast) {
+    return int(xfast ? e / unsigned(BX) : e % unsigned(BK));
+  }
+
+  __device__ void init(unsigned tid, int64_t x0, int64_t X, const Idx* xmap,
+                       bool xfast) {
+    xok = 0;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int64_t xg = x0 + xc(tid + 256u * u, xfast);
+      xoff[u] = xmap[xg < X ? xg : X - 1];
+      xok |= (xg < X ? 1u : 0u) << u;
+    }
+  }
+  __device__ void load_k(unsigned tid, int64_t p0, int64_t K, const Idx* kmap,
+                         bool xfast) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int64_t kg = p0 + kc(tid + 256u * u, xfast);
+      koff[u] = kmap[kg < K ? kg : K - 1];
+    }
+  }
+  __device__ void load_data(const T2* __restrict__ base) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) val[u] = base[xoff[u] + koff[u]];
+  }
+  // Zero out-of-range entries and conjugate if asked (sign = -1).
+  template <typename T>
+  __device__ void finish(unsigned tid, int64_t p0, int64_t K, bool xfast,
+                         T sign) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int64_t kg = p0 + kc(tid + 256u * u, xfast);
+      const bool ok = ((xok >> u) & 1u) && kg < K;
+      val[u].x = ok ? val[u].x : T(0);
+      val[u].y = ok ? sign * val[u].y : T(0);
+    }
+  }
+};
+
 template <typename Idx>
 KArgs<Idx> make_args(const GemmProblem& g, int bm, int bn) {
   KArgs<Idx> a;

@@ -173,7 +173,7 @@ def run(a_np, la, b_np, lb, c_np, lc, **kw):
 
 
 def oracle_einsum(a, la, b, lb, c0, lc, alpha=1.0, conj_a=False, conj_b=False):
-    c = np.asfortranarray(c0.copy())
+    c = c0.copy(order="F")
     orc.einsum(a, la, b, lb, c, lc, alpha=alpha, conj_a=conj_a, conj_b=conj_b)
     return c
 

@@ -21,7 +21,9 @@ def rand(rng, shape, dtype, kind="uniform"):
         x = r * np.exp(1j * th)
     else:
         raise ValueError(kind)
-    return np.asfortranarray(x.astype(dtype))
+    x = np.asarray(x).astype(dtype)
+    # np.asfortranarray turns a 0-d array into shape (1,); keep scalars 0-d
+    return x if x.ndim == 0 else np.asfortranarray(x)
 
 
 def to_dev(x, device="cuda"):

@@ -51,20 +51,21 @@ __device__ inline f32x4 mfma_bf16(const u32x4& a, const u32x4& b, f32x4 c) {
       __builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
-template <typename Idx, int BM, int BN, int BK>
-__global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(1, 1)))
+template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
+__global__ void __launch_bounds__(64 * WGM * WGN)
 k_contract_bf16x3(KArgs<Idx> p) {
-  constexpr int MB = BM / 16;
+  constexpr int NT = 64 * WGM * WGN;
+  constexpr int WTM = BM / WGM;
+  constexpr int MB = WTM / 8;
   constexpr int NBT = BN / 16;
-  constexpr int NBW = NBT / 2;
-  constexpr bool XT = (NBT % 2) != 0;
-  constexpr int XB = XT ? (MB + 1) / 2 : 0;
+  constexpr int NBW = NBT / WGN;
+  constexpr bool XT = NBT % WGN != 0;
+  constexpr int XB = XT ? (MB + WGN - 1) / WGN : 0;
   constexpr int XB_ALLOC = XT ? XB : 1;
   constexpr int PITCH = BK + 4;          // words per LDS row
   static_assert(BK % 16 == 0, "16 complex k per MFMA");
-  static_assert(BM % 16 == 0 && BN % 16 == 0, "tile shape");
-  static_assert((BM * BK) % 256 == 0 && (BK * BN) % 256 == 0, "staging");
+  static_assert(WTM % 8 == 0 && BN % 16 == 0, "tile shape");
+  static_assert(NBT - NBW * WGN <= 1, "at most one shared column block");
 
   __shared__ __attribute__((aligned(16))) uint32_t sAh[2][BM][PITCH];
   __shared__ __attribute__((aligned(16))) uint32_t sAl[2][BM][PITCH];
@@ -73,44 +74,47 @@ k_contract_bf16x3(KArgs<Idx> p) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
-  const int wm = wid & 1, wn = wid >> 1;
+  const int wm = wid % WGM, wn = wid / WGM;
   int tm, tn;
   tile_coords(p.tiles_m, p.tiles_n, tm, tn);
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
-  const int64_t M = p.M, N = p.N, K = p.K;
+  const int64_t M = p.M, N = p.N;
+  const int K = int(p.K);
   const float2* __restrict__ A = static_cast<const float2*>(p.A);
   const float2* __restrict__ B = static_cast<const float2*>(p.B);
 
   const unsigned utid = unsigned(tid);
-  const bool afast = p.a_mfast != 0;   // A lanes along m
-  const bool bfast = p.b_kfast == 0;   // B lanes along n
-  TileGather<float2, Idx, BM, BK> ga;
-  TileGather<float2, Idx, BN, BK> gb;
-  ga.init(utid, i0, M, p.rsA, afast);
-  gb.init(utid, j0, N, p.csB, bfast);
+  using GA = TileGather<float2, Idx, BM, BK, NT, AF>;
+  using GB = TileGather<float2, Idx, BN, BK, NT, BF>;
+  GA ga;
+  GB gb;
+  ga.init(utid, i0, M, p.rsA);
+  gb.init(utid, j0, N, p.csB);
   const float sa = p.conj_a ? -1.f : 1.f;
   const float sb = p.conj_b ? -1.f : 1.f;
 
-  auto swrite = [&](int buf, int64_t p0) {
-    ga.finish(utid, p0, K, afast, sa);
-    gb.finish(utid, p0, K, bfast, sb);
+  auto swrite = [&](int buf, int p0) {
+    ga.finish(utid, p0, K, sa);
+    gb.finish(utid, p0, K, sb);
 #pragma unroll
-    for (int u = 0; u < ga.E; ++u) {
-      const unsigned e = utid + 256u * u;
+    for (int u = 0; u < GA::E; ++u) {
+      const unsigned e = GA::elem(utid, u);
       uint32_t hi, lo;
       split_pair(ga.val[u].x, ga.val[u].y, hi, lo);
-      const int m = ga.xc(e, afast), k = ga.kc(e, afast);
-      sAh[buf][m][k] = hi;
-      sAl[buf][m][k] = lo;
+      if (GA::valid(utid, u)) {
+        sAh[buf][GA::xc(e)][GA::kc(e)] = hi;
+        sAl[buf][GA::xc(e)][GA::kc(e)] = lo;
+      }
     }
 #pragma unroll
-    for (int u = 0; u < gb.E; ++u) {
-      const unsigned e = utid + 256u * u;
+    for (int u = 0; u < GB::E; ++u) {
+      const unsigned e = GB::elem(utid, u);
       uint32_t hi, lo;
       split_pair(gb.val[u].x, gb.val[u].y, hi, lo);
-      const int n = gb.xc(e, bfast), k = gb.kc(e, bfast);
-      sBh[buf][n][k] = hi;
-      sBl[buf][n][k] = lo;
+      if (GB::valid(utid, u)) {
+        sBh[buf][GB::xc(e)][GB::kc(e)] = hi;
+        sBl[buf][GB::xc(e)][GB::kc(e)] = lo;
+      }
     }
   };
 
@@ -126,9 +130,15 @@ k_contract_bf16x3(KArgs<Idx> p) {
   const int g = lane >> 4;
   const int l16 = lane & 15;
   const bool odd = (lane & 1) != 0;
-  const int am0 = wm * (BM / 2) + (l16 >> 1);
-  const int bn0 = wn * (NBW + (XT ? 1 : 0)) * 16 + l16;
-  const int bnx = NBW * 16 + l16;
+  const int am0 = wm * WTM + (l16 >> 1);
+  const int bn0 = wn * NBW * 16 + l16;
+  const int bnx = NBW * WGN * 16 + l16;
+  int xmb[XB_ALLOC];
+#pragma unroll
+  for (int x = 0; x < XB_ALLOC; ++x) {
+    const int mb = wn * XB + x;
+    xmb[x] = mb < MB ? mb : MB - 1;
+  }
   // 1M expansion of a packed (re, im) word for real row parity s:
   // s = 0 -> (re, -im), s = 1 -> (im, re).
   const uint32_t psel = odd ? 0x01000302u : 0x03020100u;
@@ -139,60 +149,59 @@ k_contract_bf16x3(KArgs<Idx> p) {
     for (int t = 0; t < 4; ++t) r[t] = __builtin_amdgcn_perm(w[t], w[t], psel) ^ pxor;
     return r;
   };
+  auto ldsA = [&](const uint32_t (*plane)[PITCH], int row, int kw) {
+    return expand(*reinterpret_cast<const u32x4*>(&plane[row][kw]));
+  };
+  auto ldsB = [&](const uint32_t (*plane)[PITCH], int row, int kw) {
+    return *reinterpret_cast<const u32x4*>(&plane[row][kw]);
+  };
 
+  // B fragments stay in registers for a k-chunk; A fragments stream per row
+  // block.  Three products per tile: lo*hi and hi*lo first, then hi*hi.
   auto compute = [&](int buf) {
 #pragma unroll
     for (int kc = 0; kc < BK / 16; ++kc) {
       const int kw = kc * 16 + 4 * g;
-      u32x4 ah[MB], al[MB];
-#pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-        ah[mb] = expand(*reinterpret_cast<const u32x4*>(&sAh[buf][am0 + mb * 8][kw]));
-        al[mb] = expand(*reinterpret_cast<const u32x4*>(&sAl[buf][am0 + mb * 8][kw]));
-      }
+      u32x4 bh[NBW], bl[NBW];
 #pragma unroll
       for (int nb = 0; nb < NBW; ++nb) {
-        const u32x4 bh = *reinterpret_cast<const u32x4*>(&sBh[buf][bn0 + nb * 16][kw]);
-        const u32x4 bl = *reinterpret_cast<const u32x4*>(&sBl[buf][bn0 + nb * 16][kw]);
+        bh[nb] = ldsB(sBh[buf], bn0 + nb * 16, kw);
+        bl[nb] = ldsB(sBl[buf], bn0 + nb * 16, kw);
+      }
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) {
-          acc[mb][nb] = mfma_bf16(bl, ah[mb], acc[mb][nb]);
-          acc[mb][nb] = mfma_bf16(bh, al[mb], acc[mb][nb]);
-          acc[mb][nb] = mfma_bf16(bh, ah[mb], acc[mb][nb]);
+      for (int mb = 0; mb < MB; ++mb) {
+        const u32x4 ah = ldsA(sAh[buf], am0 + mb * 8, kw);
+        const u32x4 al = ldsA(sAl[buf], am0 + mb * 8, kw);
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) {
+          acc[mb][nb] = mfma_bf16(bl[nb], ah, acc[mb][nb]);
+          acc[mb][nb] = mfma_bf16(bh[nb], al, acc[mb][nb]);
+          acc[mb][nb] = mfma_bf16(bh[nb], ah, acc[mb][nb]);
         }
       }
       if constexpr (XT) {
-        const u32x4 bh = *reinterpret_cast<const u32x4*>(&sBh[buf][bnx][kw]);
-        const u32x4 bl = *reinterpret_cast<const u32x4*>(&sBl[buf][bnx][kw]);
-        if (wn == 0) {
+        const u32x4 xbh = ldsB(sBh[buf], bnx, kw);
+        const u32x4 xbl = ldsB(sBl[buf], bnx, kw);
 #pragma unroll
-          for (int x = 0; x < XB; ++x) {
-            accx[x] = mfma_bf16(bl, ah[x], accx[x]);
-            accx[x] = mfma_bf16(bh, al[x], accx[x]);
-            accx[x] = mfma_bf16(bh, ah[x], accx[x]);
-          }
-        } else {
-#pragma unroll
-          for (int x = 0; x < XB; ++x) {
-            constexpr int kLast = MB - 1;
-            const int mb = (XB + x) < MB ? (XB + x) : kLast;
-            accx[x] = mfma_bf16(bl, ah[mb], accx[x]);
-            accx[x] = mfma_bf16(bh, al[mb], accx[x]);
-            accx[x] = mfma_bf16(bh, ah[mb], accx[x]);
-          }
+        for (int x = 0; x < XB; ++x) {
+          const u32x4 ah = ldsA(sAh[buf], am0 + xmb[x] * 8, kw);
+          const u32x4 al = ldsA(sAl[buf], am0 + xmb[x] * 8, kw);
+          accx[x] = mfma_bf16(xbl, ah, accx[x]);
+          accx[x] = mfma_bf16(xbh, al, accx[x]);
+          accx[x] = mfma_bf16(xbh, ah, accx[x]);
         }
       }
     }
   };
 
-  const int nst = int((K + BK - 1) / BK);
-  ga.load_k(utid, 0, K, p.csA, afast);
-  gb.load_k(utid, 0, K, p.rsB, bfast);
+  const int nst = (K + BK - 1) / BK;
+  ga.load_k(utid, 0, K, p.csA);
+  gb.load_k(utid, 0, K, p.rsB);
   ga.load_data(A);
   gb.load_data(B);
   if (nst > 1) {
-    ga.load_k(utid, BK, K, p.csA, afast);
-    gb.load_k(utid, BK, K, p.rsB, bfast);
+    ga.load_k(utid, BK, K, p.csA);
+    gb.load_k(utid, BK, K, p.rsB);
   }
   swrite(0, 0);
   __syncthreads();
@@ -203,11 +212,11 @@ k_contract_bf16x3(KArgs<Idx> p) {
       gb.load_data(B);
     }
     if (s + 2 < nst) {
-      ga.load_k(utid, int64_t(s + 2) * BK, K, p.csA, afast);
-      gb.load_k(utid, int64_t(s + 2) * BK, K, p.rsB, bfast);
+      ga.load_k(utid, (s + 2) * BK, K, p.csA);
+      gb.load_k(utid, (s + 2) * BK, K, p.rsB);
     }
     compute(s & 1);
-    if (more) swrite((s + 1) & 1, int64_t(s + 1) * BK);
+    if (more) swrite((s + 1) & 1, (s + 1) * BK);
     __syncthreads();
   }
 
@@ -226,7 +235,7 @@ k_contract_bf16x3(KArgs<Idx> p) {
       } else {
         val = ar * val;
       }
-      const int64_t j = jb + r;
+      const int64_t j = jb + 4 * g + r;
       if (iok && j < N) Cr[2 * int64_t(rc + p.csC[j]) + (odd ? 1 : 0)] += val;
     }
   };
@@ -234,30 +243,35 @@ k_contract_bf16x3(KArgs<Idx> p) {
   for (int mb = 0; mb < MB; ++mb) {
     const int64_t i = i0 + am0 + mb * 8;
 #pragma unroll
-    for (int nb = 0; nb < NBW; ++nb)
-      store(acc[mb][nb], i, j0 + wn * (NBW + (XT ? 1 : 0)) * 16 + nb * 16 + g * 4);
+    for (int nb = 0; nb < NBW; ++nb) store(acc[mb][nb], i, j0 + wn * NBW * 16 + nb * 16);
   }
   if constexpr (XT) {
 #pragma unroll
-    for (int x = 0; x < XB; ++x) {
-      const int mb = wn == 0 ? x : XB + x;
-      if (mb < MB) store(accx[x], i0 + am0 + mb * 8, j0 + NBW * 16 + g * 4);
-    }
+    for (int x = 0; x < XB; ++x)
+      if (wn * XB + x < MB) store(accx[x], i0 + am0 + (wn * XB + x) * 8, j0 + NBW * WGN * 16);
   }
 }
 
-template <typename Idx, int BM, int BN, int BK>
+template <typename Idx, int BM, int BN, int BK, int WGM, int WGN>
 hipError_t launch_t(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
-  const dim3 grid(unsigned(a.tiles_m * a.tiles_n)), block(256);
-  hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK>), grid, block, 0, s, a);
+  const dim3 grid(unsigned(a.tiles_m * a.tiles_n)), block(64 * WGM * WGN);
+  const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
+  if (af && bf)
+    hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, a);
+  else if (af)
+    hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, true, false>), grid, block, 0, s, a);
+  else if (bf)
+    hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, false, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, false, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
 template <typename Idx>
 hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
-  if (kernel == K_C64_BF16X3_144) return launch_t<Idx, 144, 144, 16>(g, s);
-  return launch_t<Idx, 64, 64, 16>(g, s);
+  if (kernel == K_C64_BF16X3_144) return launch_t<Idx, 144, 144, 16, 2, 4>(g, s);
+  return launch_t<Idx, 64, 64, 16, 2, 2>(g, s);
 }
 
 }  // namespace

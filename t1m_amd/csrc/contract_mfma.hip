@@ -25,11 +25,13 @@
 //     t1m.h:543-548), so arbitrarily strided C works and rows with unit stride
 //     give contiguous 64-byte runs per 16 lanes.
 //
-// Wave layout: 4 waves as 2 (m) x 2 (n).  A wave owns BM/2 complex rows,
-// i.e. MB = BM/16 blocks of 8 rows, and BN/2 columns.  When BN/16 is odd the
-// middle 16-column block is shared: each n-wave takes half of its row blocks
-// (the 144 x 144 tile: 9 blocks of 16 columns, waves own 4 + half of the
-// middle one, 41 MFMA tiles each instead of 45 for a padded split).
+// Wave layout: WGM x WGN waves.  A wave owns BM/WGM complex rows (MB blocks
+// of 8) and NBW = floor(BN/16 / WGN) blocks of 16 columns.  When BN/16 is
+// not a multiple of WGN, the one leftover 16-column block is shared: each
+// n-wave takes XB = ceil(MB/WGN) of its row blocks, reading those A
+// fragments from LDS by a runtime index.  For 144 x 144 tiles on 2 x 4 waves
+// (2 waves per SIMD) a wave computes 18 + 3 MFMA tiles, 21 against an even
+// share of 20.25.
 #include "internal.h"
 #include "kernel_common.h"
 
@@ -37,25 +39,26 @@ namespace t1m_hip {
 
 namespace {
 
-template <typename T, typename Idx, int BM, int BN, int BK>
-__global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(1, 1)))
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
+          bool AF, bool BF>
+__global__ void __launch_bounds__(64 * WGM * WGN)
 k_contract_1m(KArgs<Idx> p) {
   using C2 = typename Vec<T>::c2;
   using Acc = typename Vec<T>::acc;
-  constexpr int MB = BM / 16;            // 8-row blocks per wave
+  constexpr int NT = 64 * WGM * WGN;
+  constexpr int WTM = BM / WGM;          // complex rows per wave
+  constexpr int MB = WTM / 8;            // 8-row blocks per wave
   constexpr int NBT = BN / 16;           // 16-column blocks per workgroup
-  constexpr int NBW = NBT / 2;           // full column blocks per wave
-  constexpr bool XT = (NBT % 2) != 0;    // shared middle column block
-  constexpr int XB = XT ? (MB + 1) / 2 : 0;
+  constexpr int NBW = NBT / WGN;         // full column blocks per wave
+  constexpr bool XT = NBT % WGN != 0;    // one shared column block
+  constexpr int XB = XT ? (MB + WGN - 1) / WGN : 0;
   constexpr int XB_ALLOC = XT ? XB : 1;
   // Row pitch padding (complex elements) so the four lane groups of a
   // fragment read hit different LDS banks.
-  constexpr int PADA = sizeof(T) == 4 ? 8 : 4;
-  constexpr int PADB = sizeof(T) == 4 ? 8 : 4;
-  constexpr int LDA = BM + PADA, LDB = BN + PADB;
-  static_assert(BM % 16 == 0 && BN % 16 == 0, "tile shape");
-  static_assert((BM * BK) % 256 == 0 && (BK * BN) % 256 == 0, "staging");
+  constexpr int PAD = sizeof(T) == 4 ? 8 : 4;
+  constexpr int LDA = BM + PAD, LDB = BN + PAD;
+  static_assert(WTM % 8 == 0 && BN % 16 == 0, "tile shape");
+  static_assert(NBT - NBW * WGN <= 1, "at most one shared column block");
   static_assert(BK % 4 == 0, "k chunk");
 
   __shared__ C2 sA[2][BK][LDA];
@@ -63,38 +66,39 @@ k_contract_1m(KArgs<Idx> p) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
-  const int wm = wid & 1, wn = wid >> 1;
+  const int wm = wid % WGM, wn = wid / WGM;
   int tm, tn;
   tile_coords(p.tiles_m, p.tiles_n, tm, tn);
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
-  const int64_t M = p.M, N = p.N, K = p.K;
+  const int64_t M = p.M, N = p.N;
+  const int K = int(p.K);
 
   const C2* __restrict__ A = static_cast<const C2*>(p.A);
   const C2* __restrict__ B = static_cast<const C2*>(p.B);
 
   // ---- gathers: A tile (BM x BK), B tile (BN x BK) -----------------------
   const unsigned utid = unsigned(tid);
-  const bool afast = p.a_mfast != 0;   // A lanes along m
-  const bool bfast = p.b_kfast == 0;   // B lanes along n
-  TileGather<C2, Idx, BM, BK> ga;
-  TileGather<C2, Idx, BN, BK> gb;
-  ga.init(utid, i0, M, p.rsA, afast);
-  gb.init(utid, j0, N, p.csB, bfast);
+  using GA = TileGather<C2, Idx, BM, BK, NT, AF>;
+  using GB = TileGather<C2, Idx, BN, BK, NT, BF>;
+  GA ga;
+  GB gb;
+  ga.init(utid, i0, M, p.rsA);
+  gb.init(utid, j0, N, p.csB);
   const T sa = p.conj_a ? T(-1) : T(1);
   const T sb = p.conj_b ? T(-1) : T(1);
 
-  auto swrite = [&](int buf, int64_t p0) {
-    ga.finish(utid, p0, K, afast, sa);
-    gb.finish(utid, p0, K, bfast, sb);
+  auto swrite = [&](int buf, int p0) {
+    ga.finish(utid, p0, K, sa);
+    gb.finish(utid, p0, K, sb);
 #pragma unroll
-    for (int u = 0; u < ga.E; ++u) {
-      const unsigned e = utid + 256u * u;
-      sA[buf][ga.kc(e, afast)][ga.xc(e, afast)] = ga.val[u];
+    for (int u = 0; u < GA::E; ++u) {
+      const unsigned e = GA::elem(utid, u);
+      if (GA::valid(utid, u)) sA[buf][GA::kc(e)][GA::xc(e)] = ga.val[u];
     }
 #pragma unroll
-    for (int u = 0; u < gb.E; ++u) {
-      const unsigned e = utid + 256u * u;
-      sB[buf][gb.kc(e, bfast)][gb.xc(e, bfast)] = gb.val[u];
+    for (int u = 0; u < GB::E; ++u) {
+      const unsigned e = GB::elem(utid, u);
+      if (GB::valid(utid, u)) sB[buf][GB::kc(e)][GB::xc(e)] = gb.val[u];
     }
   };
 
@@ -110,9 +114,15 @@ k_contract_1m(KArgs<Idx> p) {
   const int g = lane >> 4;           // k sub-index inside a 4-k chunk
   const int l16 = lane & 15;
   const bool odd = (lane & 1) != 0;  // real row parity (re / im of C)
-  const int am0 = wm * (BM / 2) + (l16 >> 1);
-  const int bn0 = wn * (NBW + (XT ? 1 : 0)) * 16 + l16;
-  const int bnx = NBW * 16 + l16;
+  const int am0 = wm * WTM + (l16 >> 1);
+  const int bn0 = wn * NBW * 16 + l16;
+  const int bnx = NBW * WGN * 16 + l16;
+  int xmb[XB_ALLOC];                 // shared-block row blocks of this wave
+#pragma unroll
+  for (int x = 0; x < XB_ALLOC; ++x) {
+    const int mb = wn * XB + x;
+    xmb[x] = mb < MB ? mb : MB - 1;
+  }
 
   auto compute = [&](int buf) {
 #pragma unroll
@@ -137,20 +147,13 @@ k_contract_1m(KArgs<Idx> p) {
         }
       if constexpr (XT) {
         const C2 bx = sB[buf][q][bnx];
-        if (wn == 0) {
 #pragma unroll
-          for (int x = 0; x < XB; ++x) {
-            accx[x] = mfma16(bx.x, are[x], accx[x]);
-            accx[x] = mfma16(bx.y, aim[x], accx[x]);
-          }
-        } else {
-#pragma unroll
-          for (int x = 0; x < XB; ++x) {
-            constexpr int kLast = MB - 1;
-            const int mb = (XB + x) < MB ? (XB + x) : kLast;
-            accx[x] = mfma16(bx.x, are[mb], accx[x]);
-            accx[x] = mfma16(bx.y, aim[mb], accx[x]);
-          }
+        for (int x = 0; x < XB; ++x) {
+          const C2 a = sA[buf][q][am0 + xmb[x] * 8];
+          const T xr = odd ? a.y : a.x;
+          const T xi = odd ? a.x : -a.y;
+          accx[x] = mfma16(bx.x, xr, accx[x]);
+          accx[x] = mfma16(bx.y, xi, accx[x]);
         }
       }
     }
@@ -159,14 +162,14 @@ k_contract_1m(KArgs<Idx> p) {
   // ---- main loop: double-buffered LDS, one barrier per stage ------------
   // Iteration s multiplies stage s from LDS while the data loads of stage
   // s+1 and the k-offset loads of stage s+2 are in flight.
-  const int nst = int((K + BK - 1) / BK);
-  ga.load_k(utid, 0, K, p.csA, afast);
-  gb.load_k(utid, 0, K, p.rsB, bfast);
+  const int nst = (K + BK - 1) / BK;
+  ga.load_k(utid, 0, K, p.csA);
+  gb.load_k(utid, 0, K, p.rsB);
   ga.load_data(A);
   gb.load_data(B);
   if (nst > 1) {
-    ga.load_k(utid, BK, K, p.csA, afast);
-    gb.load_k(utid, BK, K, p.rsB, bfast);
+    ga.load_k(utid, BK, K, p.csA);
+    gb.load_k(utid, BK, K, p.rsB);
   }
   swrite(0, 0);
   __syncthreads();
@@ -177,15 +180,15 @@ k_contract_1m(KArgs<Idx> p) {
       gb.load_data(B);
     }
     if (s + 2 < nst) {
-      ga.load_k(utid, int64_t(s + 2) * BK, K, p.csA, afast);
-      gb.load_k(utid, int64_t(s + 2) * BK, K, p.rsB, bfast);
+      ga.load_k(utid, (s + 2) * BK, K, p.csA);
+      gb.load_k(utid, (s + 2) * BK, K, p.rsB);
     }
     compute(s & 1);
-    if (more) swrite((s + 1) & 1, int64_t(s + 1) * BK);
+    if (more) swrite((s + 1) & 1, (s + 1) * BK);
     __syncthreads();
   }
 
-  // ---- epilogue: C[rsC[i] + csC[j]] (+)= alpha * acc ---------------------
+  // ---- epilogue: C[rsC[i] + csC[j]] += alpha * acc -----------------------
   T* __restrict__ Cr = static_cast<T*>(p.C);
   const bool cplx_alpha = p.alpha_im != 0.0;
   const T ar = T(p.alpha_re), ai = T(p.alpha_im);
@@ -215,15 +218,12 @@ k_contract_1m(KArgs<Idx> p) {
     const int64_t i = i0 + am0 + mb * 8;
 #pragma unroll
     for (int nb = 0; nb < NBW; ++nb)
-      store(acc[mb][nb], i, j0 + wn * (NBW + (XT ? 1 : 0)) * 16 + nb * 16);
+      store(acc[mb][nb], i, j0 + wn * NBW * 16 + nb * 16);
   }
   if constexpr (XT) {
 #pragma unroll
-    for (int x = 0; x < XB; ++x) {
-      const int mb = wn == 0 ? x : XB + x;
-      if (mb < MB)
-        store(accx[x], i0 + am0 + mb * 8, j0 + NBW * 16);
-    }
+    for (int x = 0; x < XB; ++x)
+      if (wn * XB + x < MB) store(accx[x], i0 + am0 + (wn * XB + x) * 8, j0 + NBW * WGN * 16);
   }
 }
 
@@ -278,21 +278,29 @@ hipError_t launch_simple(const GemmProblem& g, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T, typename Idx, int BM, int BN, int BK>
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN>
 hipError_t launch_1m(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
-  const dim3 grid(unsigned(a.tiles_m * a.tiles_n)), block(256);
-  hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK>), grid, block, 0, s, a);
+  const dim3 grid(unsigned(a.tiles_m * a.tiles_n)), block(64 * WGM * WGN);
+  const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
+  if (af && bf)
+    hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, a);
+  else if (af)
+    hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, false>), grid, block, 0, s, a);
+  else if (bf)
+    hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
 template <typename Idx>
 hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
   switch (kernel) {
-    case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, 16>(g, s);
-    case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 16>(g, s);
-    case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16>(g, s);
-    case K_C128_F64_32: return launch_1m<double, Idx, 32, 32, 16>(g, s);
+    case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, 16, 2, 4>(g, s);
+    case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 16, 2, 2>(g, s);
+    case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16, 2, 2>(g, s);
+    case K_C128_F64_32: return launch_1m<double, Idx, 32, 32, 16, 2, 2>(g, s);
     default: return launch_simple<Idx>(g, s);
   }
 }

@@ -59,46 +59,53 @@ __device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
 
 
 // Gather of one operand tile (BX free-index values x BK contracted values)
-// from strided global memory, for 256 lanes.  Element e = tid + 256 u of the
+// from strided global memory by NT lanes.  Element e = tid + NT u of the
 // tile sits at (x, k) = (e % BX, e / BX) when lanes run along the free index
-// (xfast) and at (e / BK, e % BK) otherwise; the choice follows the
-// operand's unit-stride direction, so consecutive lanes read consecutive
-// addresses.  Loads are issued without branches: out-of-range rows / k read
-// a valid in-range element and are zeroed when the tile is written to LDS,
-// after the main loop's multiply has overlapped their latency.  The k
-// offsets of the next stage are fetched one stage ahead (load_k) so no data
-// load waits on an index load.
-template <typename T2, typename Idx, int BX, int BK>
+// (XF) and at (e / BK, e % BK) otherwise; XF follows the operand's unit-stride
+// direction, so consecutive lanes read consecutive addresses.  Loads are
+// issued without branches: out-of-range rows / k read a valid in-range
+// element and are zeroed when the tile is written to LDS, after the main
+// loop's multiply has overlapped their latency.  The k offsets of the next
+// stage are fetched one stage ahead (load_k) so no data load waits on an
+// index load.  When BX*BK is not a multiple of NT the last slot is only
+// used by the first lanes (valid()).
+template <typename T2, typename Idx, int BX, int BK, int NT, bool XF>
 struct TileGather {
-  static constexpr int E = BX * BK / 256;
-  static_assert((BX * BK) % 256 == 0, "tile must split evenly over 256 lanes");
+  static constexpr int TOTAL = BX * BK;
+  static constexpr int E = (TOTAL + NT - 1) / NT;
+  static constexpr bool RAGGED = (TOTAL % NT) != 0;
   Idx xoff[E];
   Idx koff[E];
   T2 val[E];
   uint32_t xok;
 
-  __device__ static int xc(unsigned e, bool xfast) {
-    return int(xfast ? e % unsigned(BX) : e / unsigned(BK));
+  __device__ static constexpr int xc(unsigned e) {
+    return int(XF ? e % unsigned(BX) : e / unsigned(BK));
   }
-  __device__ static int kc(unsigned e, bool xfast) {
-    return int(xfast ? e / unsigned(BX) : e % unsigned(BK));
+  __device__ static constexpr int kc(unsigned e) {
+    return int(XF ? e / unsigned(BX) : e % unsigned(BK));
+  }
+  __device__ static bool valid(unsigned tid, int u) {
+    return !RAGGED || u + 1 < E || tid + unsigned(NT) * u < unsigned(TOTAL);
+  }
+  __device__ static unsigned elem(unsigned tid, int u) {
+    // Clamp the unused tail slots onto element 0 so every address is valid.
+    return valid(tid, u) ? tid + unsigned(NT) * u : 0u;
   }
 
-  __device__ void init(unsigned tid, int64_t x0, int64_t X, const Idx* xmap,
-                       bool xfast) {
+  __device__ void init(unsigned tid, int64_t x0, int64_t X, const Idx* xmap) {
     xok = 0;
 #pragma unroll
     for (int u = 0; u < E; ++u) {
-      const int64_t xg = x0 + xc(tid + 256u * u, xfast);
+      const int64_t xg = x0 + xc(elem(tid, u));
       xoff[u] = xmap[xg < X ? xg : X - 1];
-      xok |= (xg < X ? 1u : 0u) << u;
+      xok |= ((xg < X && valid(tid, u)) ? 1u : 0u) << u;
     }
   }
-  __device__ void load_k(unsigned tid, int64_t p0, int64_t K, const Idx* kmap,
-                         bool xfast) {
+  __device__ void load_k(unsigned tid, int p0, int K, const Idx* kmap) {
 #pragma unroll
     for (int u = 0; u < E; ++u) {
-      const int64_t kg = p0 + kc(tid + 256u * u, xfast);
+      const int kg = p0 + kc(elem(tid, u));
       koff[u] = kmap[kg < K ? kg : K - 1];
     }
   }
@@ -108,11 +115,10 @@ struct TileGather {
   }
   // Zero out-of-range entries and conjugate if asked (sign = -1).
   template <typename T>
-  __device__ void finish(unsigned tid, int64_t p0, int64_t K, bool xfast,
-                         T sign) {
+  __device__ void finish(unsigned tid, int p0, int K, T sign) {
 #pragma unroll
     for (int u = 0; u < E; ++u) {
-      const int64_t kg = p0 + kc(tid + 256u * u, xfast);
+      const int kg = p0 + kc(elem(tid, u));
       const bool ok = ((xok >> u) & 1u) && kg < K;
       val[u].x = ok ? val[u].x : T(0);
       val[u].y = ok ? sign * val[u].y : T(0);

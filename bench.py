@@ -75,9 +75,9 @@ def make_inputs(cfg, kind, seed, torch):
 
 def time_path(t1m, torch, a, b, c, cfg, precision, steps, warmup):
     stream = torch.cuda.current_stream()
-    for _ in range(warmup):
+    for _ in range(max(warmup, 1)):   # the first call autotunes the tile shape
         t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision,
-                     synchronize=False)
+                     synchronize=False, autotune=True)
     torch.cuda.synchronize()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
@@ -90,7 +90,8 @@ def time_path(t1m, torch, a, b, c, cfg, precision, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kern_ms = [s.elapsed_time(e) for s, e in evs]
-    return wall, kern_ms
+    kernel = t1m.plan_kernel(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision)
+    return wall, kern_ms, kernel
 
 
 def rel_errors(t1m, torch, cfg, seed):
@@ -152,7 +153,19 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-errors", action="store_true", help="skip the error report")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
               "multi-GPU sharding is design-only.", file=sys.stderr)
@@ -165,7 +178,7 @@ def main():
     fl, (m, n, k) = flops(cfg)
     a, b, c = make_inputs(cfg, "uniform" if cfg["dtype"] == "complex64" else "normal", 2, torch)
 
-    wall, kern = time_path(t1m, torch, a, b, c, cfg, "native", args.steps, args.warmup)
+    wall, kern, kname = time_path(t1m, torch, a, b, c, cfg, "native", args.steps, args.warmup)
     ms_step = wall / args.steps * 1e3
     value = fl * args.steps / wall / 1e12
     kern_avg = sum(kern) / len(kern)
@@ -182,14 +195,15 @@ def main():
                    "m": m, "n": n, "k": k, "flop_per_step": fl, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel_ms": round(kern_avg, 4)},
+                     "kernel_ms": round(kern_avg, 4), "kernel": kname},
     }
     if cfg["dtype"] == "complex64":
         c2 = torch.zeros_like(c)
-        wall_b, kern_b = time_path(t1m, torch, a, b, c2, cfg, "bf16x3", args.steps, args.warmup)
+        wall_b, kern_b, kname_b = time_path(t1m, torch, a, b, c2, cfg, "bf16x3", args.steps,
+                                            args.warmup)
         kb = sum(kern_b) / len(kern_b)
         line["bf16x3"] = {"value": round(fl * args.steps / wall_b / 1e12, 3), "unit": "TFLOP/s",
-                          "kernel_ms": round(kb, 4),
+                          "kernel_ms": round(kb, 4), "kernel": kname_b,
                           "speedup_vs_fp32": round(kern_avg / kb, 3),
                           "frac_of_fp32_peak": round(fl / (kb * 1e-3) / 1e12 / peak, 4)}
     if not args.no_errors:

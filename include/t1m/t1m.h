@@ -34,6 +34,7 @@ struct context {
   void* stream = nullptr;                     // hipStream_t
   int precision = T1M_PRECISION_NATIVE;       // t1m_precision
   bool synchronize = true;                    // result visible on return
+  bool autotune = false;                      // time tile shapes on first use
 };
 
 namespace detail {
@@ -90,6 +91,7 @@ void run(double alpha, const tensor<T, na>& a, const std::string& la,
     o.stream = ctx->stream;
     o.precision = ctx->precision;
     o.synchronize = ctx->synchronize ? 1 : 0;
+    o.autotune = ctx->autotune ? 1 : 0;
   }
   raise(t1m_contract(&A.t, la.c_str(), &B.t, lb.c_str(), &C.t, lc.c_str(), &o));
 }

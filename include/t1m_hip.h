@@ -72,9 +72,13 @@ typedef struct t1m_options {
   int32_t conj_b;           /* use conj(B) (complex only) */
   int32_t synchronize;      /* nonzero: the result is visible on return */
   void* stream;             /* hipStream_t; NULL = the null stream */
+  int32_t autotune;         /* nonzero: the first call of a new shape times
+                               the candidate tile shapes (on a scratch C) and
+                               keeps the fastest; see t1m_autotune_cache_path */
 } t1m_options;
 
-/* alpha = 1, NATIVE, no conjugation, synchronize = 1, null stream. */
+/* alpha = 1, NATIVE, no conjugation, synchronize = 1, null stream,
+ * autotune = 0 (tile shape from the tuning cache, else a static rule). */
 void t1m_options_init(t1m_options* opt);
 
 /*
@@ -166,6 +170,15 @@ int t1m_unpack(int32_t dtype, const t1m_view* view, const void* src,
 const char* t1m_last_error(void);
 /* Library version string. */
 const char* t1m_version(void);
+/* File that keeps autotuning results across processes: $T1M_TUNE_CACHE, else
+ * $HOME/.cache/t1m_mi355x/tuning.txt.  Lines are "<key> <kernel name>". */
+const char* t1m_autotune_cache_path(void);
+/* Name of the kernel the plan for this contraction uses (tuning / tests);
+ * builds the plan if needed.  Returns NULL on error. */
+const char* t1m_plan_kernel(const t1m_tensor* a, const char* la,
+                            const t1m_tensor* b, const char* lb,
+                            const t1m_tensor* c, const char* lc,
+                            const t1m_options* opt);
 /* Number of plans in the plan cache (index maps kept on the device). */
 int64_t t1m_plan_cache_size(void);
 /* Drop every cached plan and free its device memory. */

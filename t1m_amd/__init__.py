@@ -36,7 +36,7 @@ class Options(ctypes.Structure):
     _fields_ = [("alpha_re", ctypes.c_double), ("alpha_im", ctypes.c_double),
                 ("precision", ctypes.c_int32), ("conj_a", ctypes.c_int32),
                 ("conj_b", ctypes.c_int32), ("synchronize", ctypes.c_int32),
-                ("stream", ctypes.c_void_p)]
+                ("stream", ctypes.c_void_p), ("autotune", ctypes.c_int32)]
 
 
 class View(ctypes.Structure):
@@ -60,7 +60,7 @@ EXPORTS = ["t1m_options_init", "t1m_contract", "t1m_contract_vjp",
            "t1m_batch_create", "t1m_batch_launch", "t1m_batch_destroy",
            "t1m_index_bundles", "t1m_index_maps", "t1m_pack_block", "t1m_unpack",
            "t1m_last_error", "t1m_version", "t1m_plan_cache_size",
-           "t1m_plan_cache_clear"]
+           "t1m_plan_cache_clear", "t1m_autotune_cache_path", "t1m_plan_kernel"]
 
 _lib = None
 
@@ -99,6 +99,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.t1m_version.restype = ctypes.c_char_p
     L.t1m_plan_cache_size.restype = ctypes.c_int64
     L.t1m_plan_cache_clear.restype = None
+    L.t1m_autotune_cache_path.restype = ctypes.c_char_p
+    L.t1m_plan_kernel.argtypes = [P(TensorDesc), ctypes.c_char_p] * 3 + [P(Options)]
+    L.t1m_plan_kernel.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -135,7 +138,7 @@ class _Desc:
 
 
 def _options(alpha=1.0, precision="native", conj_a=False, conj_b=False,
-             stream=None, synchronize=True) -> Options:
+             stream=None, synchronize=True, autotune=False) -> Options:
     o = Options()
     load().t1m_options_init(ctypes.byref(o))
     al = complex(alpha)
@@ -144,6 +147,7 @@ def _options(alpha=1.0, precision="native", conj_a=False, conj_b=False,
     o.conj_a, o.conj_b = int(bool(conj_a)), int(bool(conj_b))
     o.synchronize = int(bool(synchronize))
     o.stream = _stream_handle(stream)
+    o.autotune = int(bool(autotune))
     return o
 
 
@@ -163,13 +167,14 @@ def _stream_handle(stream) -> Optional[int]:
 
 def contract(a, la: str, b, lb: str, c, lc: str, alpha=1.0, precision="native",
              conj_a: bool = False, conj_b: bool = False, stream=None,
-             synchronize: bool = True):
+             synchronize: bool = True, autotune: bool = False):
     """C += alpha * A . B over labels (t1m::contract, t1m.h:560-577).
 
     a, b, c are torch tensors of one dtype (float32/64, complex64/128) with any
-    strides; device tensors run without copies.  Returns c."""
+    strides; device tensors run without copies.  autotune=True times the
+    candidate tile shapes the first time a shape is seen.  Returns c."""
     da, db, dc = _Desc(a), _Desc(b), _Desc(c)
-    o = _options(alpha, precision, conj_a, conj_b, stream, synchronize)
+    o = _options(alpha, precision, conj_a, conj_b, stream, synchronize, autotune)
     _check(load().t1m_contract(ctypes.byref(da.desc), la.encode(), ctypes.byref(db.desc),
                                lb.encode(), ctypes.byref(dc.desc), lc.encode(),
                                ctypes.byref(o)))
@@ -278,6 +283,23 @@ def unpack(dtype: int, br: int, bc: int, rs, cs, rbs, cbs, src, dest) -> None:
     _check(load().t1m_unpack(dtype, ctypes.byref(v), src.data_ptr(), dest.data_ptr(),
                              _stream_handle(None)))
     torch.cuda.synchronize()
+
+
+def plan_kernel(a, la: str, b, lb: str, c, lc: str, precision="native",
+                autotune: bool = False) -> str:
+    """Name of the kernel the plan for this contraction uses."""
+    da, db, dc = _Desc(a), _Desc(b), _Desc(c)
+    o = _options(precision=precision, autotune=autotune)
+    r = load().t1m_plan_kernel(ctypes.byref(da.desc), la.encode(), ctypes.byref(db.desc),
+                               lb.encode(), ctypes.byref(dc.desc), lc.encode(),
+                               ctypes.byref(o))
+    if r is None:
+        raise T1MError(-1, load().t1m_last_error().decode())
+    return r.decode()
+
+
+def autotune_cache_path() -> str:
+    return load().t1m_autotune_cache_path().decode()
 
 
 def plan_cache_size() -> int:

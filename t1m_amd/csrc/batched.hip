@@ -62,6 +62,7 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
     g.conj_a = cplx && o.conj_a; g.conj_b = cplx && o.conj_b;
     g.a_mfast = plan->a_mfast; g.b_kfast = plan->b_kfast;
     g.idx32 = plan->idx32; g.dtype = P.dtype;
+    if (o.autotune && !plan->tuned.load() && (rc = autotune_plan(*plan, g, stream))) return rc;
     bt->plans.push_back(plan);
     bt->problems.push_back(g);
   }

@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -44,7 +46,11 @@ struct Problem {
 // Device-resident index maps for one (dtype, labels, extents, strides).
 struct Plan {
   int dtype = 0;
-  int kernel = 0;
+  int precision = 0;
+  std::atomic<int> kernel{0};
+  std::atomic<bool> tuned{false};
+  std::mutex tune_mu;
+  int64_t c_lo = 0, c_hi = 0;                    // C's element span
   int64_t M = 0, N = 0, K = 0;
   int a_mfast = 1, b_kfast = 1, idx32 = 1;
   void* maps = nullptr;
@@ -79,6 +85,11 @@ int get_plan(const Problem& P, int precision, hipStream_t stream,
 int64_t plan_cache_size();
 void plan_cache_clear();
 bool is_device_pointer(const void* ptr);
+// Kernel ids worth timing for a plan's dtype and precision.
+std::vector<int> kernel_candidates(int dtype, int precision);
+// Times the candidates on a scratch copy of C's extent and stores the
+// fastest in the plan (and in the tuning cache file).
+int autotune_plan(Plan& plan, const GemmProblem& g, hipStream_t stream);
 int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
                   const char* lb, const t1m_tensor* c, const char* lc,
                   const t1m_options& opt);

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <sys/stat.h>
 #include <list>
 #include <memory>
 #include <mutex>
@@ -193,28 +194,182 @@ double tile_cost(int64_t M, int64_t N, int bm, int bn, int occ, double eff) {
 
 }  // namespace
 
+std::vector<int> kernel_candidates(int dtype, int precision) {
+  if (precision == T1M_PRECISION_SIMPLE) return {K_SIMPLE};
+  switch (dtype) {
+    case T1M_COMPLEX64:
+      if (precision == T1M_PRECISION_BF16X3) return {K_C64_BF16X3_144, K_C64_BF16X3_64};
+      return {K_C64_F32_144, K_C64_F32_64};
+    case T1M_COMPLEX128:
+      return {K_C128_F64_64, K_C128_F64_32};
+    default:
+      return {K_SIMPLE};
+  }
+}
+
+// Static rule, used when no tuning result is known: the large tiles (one
+// workgroup of 2 waves per SIMD per CU) once they nearly fill the CUs,
+// otherwise the small tiles (several workgroups per CU).  Calibrated on the
+// 48^4 and 256x4x256 workloads.
 int choose_kernel(const Problem& P, int precision) {
   if (const char* f = getenv("T1M_FORCE_KERNEL")) {
     const int k = atoi(f);
     if (k >= 0 && k < K_COUNT) return k;
   }
   if (precision == T1M_PRECISION_SIMPLE) return K_SIMPLE;
+  if (P.K >= (int64_t(1) << 31)) return K_SIMPLE;     // MFMA kernels use int k
+  auto tiles = [&](int bm, int bn) {
+    return double((P.M + bm - 1) / bm) * double((P.N + bn - 1) / bn);
+  };
   switch (P.dtype) {
     case T1M_COMPLEX64: {
-      const bool bf = precision == T1M_PRECISION_BF16X3;
-      const double c144 = tile_cost(P.M, P.N, 144, 144, 1, 0.90);
-      const double c64 = tile_cost(P.M, P.N, 64, 64, 2, 0.55);
-      if (bf) return c144 <= c64 ? K_C64_BF16X3_144 : K_C64_BF16X3_64;
-      return c144 <= c64 ? K_C64_F32_144 : K_C64_F32_64;
+      const bool big = tiles(144, 144) >= 0.75 * cu_count();
+      if (precision == T1M_PRECISION_BF16X3) return big ? K_C64_BF16X3_144 : K_C64_BF16X3_64;
+      return big ? K_C64_F32_144 : K_C64_F32_64;
     }
-    case T1M_COMPLEX128: {
-      const double c64 = tile_cost(P.M, P.N, 64, 64, 1, 0.80);
-      const double c32 = tile_cost(P.M, P.N, 32, 32, 2, 0.50);
-      return c64 <= c32 ? K_C128_F64_64 : K_C128_F64_32;
-    }
+    case T1M_COMPLEX128:
+      return tiles(64, 64) >= 2.0 * cu_count() ? K_C128_F64_64 : K_C128_F64_32;
     default:
       return K_SIMPLE;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Autotuning
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct TuneCache {
+  std::mutex mu;
+  bool loaded = false;
+  std::unordered_map<std::string, int> best;   // key -> kernel id
+  std::string path;
+};
+
+TuneCache& tune_cache() {
+  static TuneCache* t = new TuneCache();
+  return *t;
+}
+
+std::string tune_path() {
+  if (const char* p = getenv("T1M_TUNE_CACHE")) return p;
+  const char* home = getenv("HOME");
+  return std::string(home ? home : "/tmp") + "/.cache/t1m_mi355x/tuning.txt";
+}
+
+int kernel_by_name(const std::string& name) {
+  for (int k = 0; k < K_COUNT; ++k)
+    if (name == kernel_info(k).name) return k;
+  return -1;
+}
+
+void load_tune_cache(TuneCache& T) {
+  if (T.loaded) return;
+  T.loaded = true;
+  T.path = tune_path();
+  FILE* f = fopen(T.path.c_str(), "r");
+  if (!f) return;
+  char key[512], name[128];
+  while (fscanf(f, "%511s %127s", key, name) == 2) {
+    const int k = kernel_by_name(name);
+    if (k >= 0) T.best[key] = k;
+  }
+  fclose(f);
+}
+
+void append_tune_cache(TuneCache& T, const std::string& key, int kernel) {
+  const std::string dir = T.path.substr(0, T.path.find_last_of('/'));
+  std::string cmd;
+  // Create the directory without a shell: mkdir each prefix.
+  for (size_t i = 1; i <= dir.size(); ++i)
+    if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+  FILE* f = fopen(T.path.c_str(), "a");
+  if (!f) return;
+  fprintf(f, "%s %s\n", key.c_str(), kernel_info(kernel).name);
+  fclose(f);
+}
+
+std::string tune_key(const Plan& pl) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  std::string arch = "gpu";
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) arch = prop.gcnArchName;
+  for (char& ch : arch) if (ch == ':' || ch == ' ') ch = '_';
+  char buf[256];
+  snprintf(buf, sizeof(buf), "%s/%d|dt%d|p%d|M%lld|N%lld|K%lld|a%d|b%d|i%d", arch.c_str(),
+           cu_count(), pl.dtype, pl.precision, (long long)pl.M, (long long)pl.N,
+           (long long)pl.K, pl.a_mfast, pl.b_kfast, pl.idx32);
+  return buf;
+}
+
+}  // namespace
+
+int autotune_plan(Plan& pl, const GemmProblem& g0, hipStream_t stream) {
+  std::lock_guard<std::mutex> guard(pl.tune_mu);
+  if (pl.tuned.load()) return T1M_SUCCESS;
+  const std::vector<int> cands = kernel_candidates(pl.dtype, pl.precision);
+  if (cands.size() < 2 || getenv("T1M_FORCE_KERNEL")) {
+    pl.tuned = true;
+    return T1M_SUCCESS;
+  }
+  TuneCache& T = tune_cache();
+  const std::string key = tune_key(pl);
+  {
+    std::lock_guard<std::mutex> lock(T.mu);
+    load_tune_cache(T);
+    auto it = T.best.find(key);
+    if (it != T.best.end()) {
+      pl.kernel = it->second;
+      pl.tuned = true;
+      return T1M_SUCCESS;
+    }
+  }
+  const int es = dtype_size(pl.dtype);
+  const size_t bytes = size_t(pl.c_hi - pl.c_lo + 1) * es;
+  char* scratch = nullptr;
+  if (hipMalloc(&scratch, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(T1M_ERROR_OUT_OF_MEMORY, "hipMalloc of %zu bytes for autotuning failed", bytes);
+  }
+  GemmProblem g = g0;
+  g.C = scratch - pl.c_lo * es;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int best = pl.kernel;
+  float best_ms = 1e30f;
+  for (int k : cands) {
+    hipError_t e = launch_contract(g, k, stream);          // warm-up
+    float kbest = 1e30f;
+    for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+      e = hipEventRecord(e0, stream);
+      if (e == hipSuccess) e = launch_contract(g, k, stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e == hipSuccess && ms < kbest) kbest = ms;
+    }
+    if (e != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipFree(scratch);
+      return set_error(T1M_ERROR_DEVICE, "autotuning %s failed: %s", kernel_info(k).name,
+                       hipGetErrorString(e));
+    }
+    if (kbest < best_ms) { best_ms = kbest; best = k; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(scratch);
+  pl.kernel = best;
+  pl.tuned = true;
+  std::lock_guard<std::mutex> lock(T.mu);
+  T.best[key] = best;
+  append_tune_cache(T, key, best);
+  return T1M_SUCCESS;
 }
 
 // ---------------------------------------------------------------------------
@@ -319,7 +474,12 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
   Plan& pl = *plan;
   pl.M = P.M; pl.N = P.N; pl.K = P.K;
   pl.dtype = P.dtype;
+  pl.precision = precision;
   pl.kernel = choose_kernel(P, precision);
+  for (size_t i = 0; i < P.dc.size(); ++i) {
+    const int64_t span = (P.dc[i] > 0 ? P.dc[i] - 1 : 0) * P.sc[i];
+    if (span < 0) pl.c_lo += span; else pl.c_hi += span;
+  }
 
   // Staging directions and traversal order of each bundle.
   pl.a_mfast = min_stride(P.m, 0) <= min_stride(P.k, 0) ? 1 : 0;
@@ -371,6 +531,16 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
     return set_error(T1M_ERROR_DEVICE, "index map construction failed: %s", hipGetErrorString(e));
   }
   pl.modes_m = P.m; pl.modes_n = P.n; pl.modes_k = P.k;
+  if (!getenv("T1M_FORCE_KERNEL")) {     // a tuning result from an earlier run
+    TuneCache& T = tune_cache();
+    std::lock_guard<std::mutex> lock(T.mu);
+    load_tune_cache(T);
+    auto it = T.best.find(tune_key(pl));
+    if (it != T.best.end()) {
+      pl.kernel = it->second;
+      pl.tuned = true;
+    }
+  }
 
   std::lock_guard<std::mutex> lock(C.mu);
   auto it = C.map.find(key);
@@ -500,10 +670,11 @@ int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
   g.a_mfast = plan->a_mfast; g.b_kfast = plan->b_kfast;
   g.idx32 = plan->idx32;
   g.dtype = P.dtype;
+  if (opt.autotune && !plan->tuned.load() && (rc = autotune_plan(*plan, g, stream))) return rc;
   hipError_t e = launch_contract(g, plan->kernel, stream);
   if (e != hipSuccess)
     return set_error(T1M_ERROR_DEVICE, "kernel %s launch failed: %s",
-                     kernel_info(plan->kernel).name, hipGetErrorString(e));
+                     kernel_info(plan->kernel.load()).name, hipGetErrorString(e));
   if (hostC && (rc = sc.copy_back(c->data, stream))) return rc;
   if (opt.synchronize || hostA || hostB || hostC) HIP_TRY(hipStreamSynchronize(stream));
   return T1M_SUCCESS;
@@ -528,6 +699,7 @@ void t1m_options_init(t1m_options* opt) {
   opt->conj_b = 0;
   opt->synchronize = 1;
   opt->stream = nullptr;
+  opt->autotune = 0;
 }
 
 const char* t1m_last_error(void) { return last_error(); }
@@ -535,6 +707,26 @@ const char* t1m_last_error(void) { return last_error(); }
 const char* t1m_version(void) { return "t1m-mi355x 0.1.0 (gfx950)"; }
 
 int64_t t1m_plan_cache_size(void) { return plan_cache_size(); }
+
+const char* t1m_autotune_cache_path(void) {
+  static thread_local std::string p;
+  p = tune_path();
+  return p.c_str();
+}
+
+const char* t1m_plan_kernel(const t1m_tensor* a, const char* la, const t1m_tensor* b,
+                            const char* lb, const t1m_tensor* c, const char* lc,
+                            const t1m_options* opt) {
+  clear_error();
+  t1m_options o;
+  if (opt) o = *opt; else t1m_options_init(&o);
+  Problem P;
+  if (parse_problem(a, la, b, lb, c, lc, P)) return nullptr;
+  if (P.M == 0 || P.N == 0 || P.K == 0) return "none";
+  std::shared_ptr<Plan> plan;
+  if (get_plan(P, o.precision, static_cast<hipStream_t>(o.stream), plan)) return nullptr;
+  return kernel_info(plan->kernel.load()).name;
+}
 
 void t1m_plan_cache_clear(void) { plan_cache_clear(); }
 

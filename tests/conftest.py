@@ -10,6 +10,10 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+    # Autotuning results of this test session go to a private file.
+    import tempfile
+    os.environ.setdefault("T1M_TUNE_CACHE",
+                          os.path.join(tempfile.mkdtemp(prefix="t1m_tune_"), "tuning.txt"))
 
 
 @pytest.fixture(scope="session")

@@ -339,6 +339,26 @@ def test_host_pointers_are_staged():
     assert rel_err(C - C0, want - C0) < TOL["fp32"]
 
 
+def test_autotune_picks_a_candidate_and_persists():
+    rng = np.random.default_rng(21)
+    la, lb, lc = "abcd", "dbef", "acef"
+    ext = {"a": 24, "b": 10, "c": 24, "d": 12, "e": 24, "f": 24}
+    A = rand(rng, [ext[x] for x in la], np.complex64)
+    B = rand(rng, [ext[x] for x in lb], np.complex64)
+    C0 = rand(rng, [ext[x] for x in lc], np.complex64)
+    want = oracle_einsum(A, la, B, lb, C0, lc)
+    for prec, cands in (("native", ("c64_f32mfma_144x144", "c64_f32mfma_64x64")),
+                        ("bf16x3", ("c64_bf16x3_144x144", "c64_bf16x3_64x64"))):
+        a, b, c = to_dev(A), to_dev(B), to_dev(C0)
+        t1m_amd.contract(a, la, b, lb, c, lc, precision=prec, autotune=True)
+        assert rel_err(to_host(c) - C0, want - C0) < TOL["fp32" if prec == "native" else "bf16x3"]
+        assert t1m_amd.plan_kernel(a, la, b, lb, c, lc, precision=prec) in cands
+    path = t1m_amd.autotune_cache_path()
+    with open(path) as f:
+        lines = [ln.split() for ln in f if ln.strip()]
+    assert any("M576|N576|K120" in ln[0] for ln in lines), lines
+
+
 def test_plan_cache_reuse():
     rng = np.random.default_rng(12)
     A, B = rand(rng, (8, 9), np.complex64), rand(rng, (9, 10), np.complex64)

@@ -145,6 +145,16 @@ def cpu_baseline(cfg, seconds_hint=15.0):
                        f"{dt_s:.2f} s, 1 thread)")}
 
 
+def traffic_bytes(config, kernel):
+    """HBM bytes per launch measured for this kernel on this workload by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f)["bytes_per_launch"].get(f"{config}|{kernel}")
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,7 +204,8 @@ def main():
         "config": {"workload": cfg["workload"], "labels": f"{cfg['la']},{cfg['lb']}->{cfg['lc']}",
                    "m": m, "n": n, "k": k, "flop_per_step": fl, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                     "traffic": traffic_bytes(args.config, kname),
                      "kernel_ms": round(kern_avg, 4), "kernel": kname},
     }
     if cfg["dtype"] == "complex64":

@@ -52,8 +52,7 @@ __device__ inline f32x4 mfma_bf16(const u32x4& a, const u32x4& b, f32x4 c) {
 }
 
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
-__global__ void __launch_bounds__(64 * WGM * WGN)
-k_contract_bf16x3(KArgs<Idx> p) {
+__device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm, int tn) {
   constexpr int NT = 64 * WGM * WGN;
   constexpr int WTM = BM / WGM;
   constexpr int MB = WTM / 8;
@@ -75,8 +74,6 @@ k_contract_bf16x3(KArgs<Idx> p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int wm = wid % WGM, wn = wid / WGM;
-  int tm, tn;
-  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
   const int64_t M = p.M, N = p.N;
   const int K = int(p.K);
@@ -252,6 +249,26 @@ k_contract_bf16x3(KArgs<Idx> p) {
   }
 }
 
+template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
+__global__ void __launch_bounds__(64 * WGM * WGN)
+k_contract_bf16x3(KArgs<Idx> p) {
+  int tm, tn;
+  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+  contract_tile_bf16x3<Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, tm, tn);
+}
+
+template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
+__global__ void __launch_bounds__(64 * WGM * WGN)
+k_contract_bf16x3_batched(const KArgs<Idx>* __restrict__ inst,
+                          const BatchItem* __restrict__ items, int nitems) {
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const BatchItem w = items[it];
+    const KArgs<Idx> p = inst[w.inst];
+    contract_tile_bf16x3<Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn);
+    __syncthreads();
+  }
+}
+
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN>
 hipError_t launch_t(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
@@ -278,6 +295,59 @@ hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
 
 hipError_t launch_bf16x3(const GemmProblem& g, int32_t kernel, hipStream_t s) {
   return g.idx32 ? dispatch<int32_t>(g, kernel, s) : dispatch<int64_t>(g, kernel, s);
+}
+
+namespace {
+
+template <typename Idx, int BM, int BN, int BK, int WGM, int WGN>
+struct BatchedBf {
+  using K = void (*)(const KArgs<Idx>*, const BatchItem*, int);
+  static K pick(bool af, bool bf) {
+    if (af && bf) return k_contract_bf16x3_batched<Idx, BM, BN, BK, WGM, WGN, true, true>;
+    if (af) return k_contract_bf16x3_batched<Idx, BM, BN, BK, WGM, WGN, true, false>;
+    if (bf) return k_contract_bf16x3_batched<Idx, BM, BN, BK, WGM, WGN, false, true>;
+    return k_contract_bf16x3_batched<Idx, BM, BN, BK, WGM, WGN, false, false>;
+  }
+  static int per_cu(bool af, bool bf) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pick(af, bf), 64 * WGM * WGN, 0) !=
+        hipSuccess) {
+      (void)hipGetLastError();
+      return 1;
+    }
+    return n > 0 ? n : 1;
+  }
+  static hipError_t launch(bool af, bool bf, const void* inst, const BatchItem* items,
+                           int nitems, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(pick(af, bf), dim3(grid), dim3(64 * WGM * WGN), 0, s,
+                       static_cast<const KArgs<Idx>*>(inst), items, nitems);
+    return hipGetLastError();
+  }
+};
+
+template <typename Idx, typename F>
+auto with_bf_config(int32_t kernel, F&& f) {
+  if (kernel == K_C64_BF16X3_144) return f(BatchedBf<Idx, 144, 144, 16, 2, 4>());
+  return f(BatchedBf<Idx, 64, 64, 16, 2, 2>());
+}
+
+}  // namespace
+
+int bf16x3_batched_blocks_per_cu(int32_t kernel, bool idx32, bool af, bool bf) {
+  if (idx32) return with_bf_config<int32_t>(kernel, [&](auto c) { return c.per_cu(af, bf); });
+  return with_bf_config<int64_t>(kernel, [&](auto c) { return c.per_cu(af, bf); });
+}
+
+hipError_t launch_bf16x3_batched(int32_t kernel, bool idx32, bool af, bool bf,
+                                 const void* inst, const BatchItem* items, int nitems,
+                                 int grid, hipStream_t s) {
+  if (idx32)
+    return with_bf_config<int32_t>(kernel, [&](auto c) {
+      return c.launch(af, bf, inst, items, nitems, grid, s);
+    });
+  return with_bf_config<int64_t>(kernel, [&](auto c) {
+    return c.launch(af, bf, inst, items, nitems, grid, s);
+  });
 }
 
 }  // namespace t1m_hip

@@ -35,14 +35,16 @@
 #include "internal.h"
 #include "kernel_common.h"
 
+#include <cstring>
+
 namespace t1m_hip {
 
 namespace {
 
+// One BM x BN tile (tm, tn) of one contraction, whole K range.
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
           bool AF, bool BF>
-__global__ void __launch_bounds__(64 * WGM * WGN)
-k_contract_1m(KArgs<Idx> p) {
+__device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, int tn) {
   using C2 = typename Vec<T>::c2;
   using Acc = typename Vec<T>::acc;
   constexpr int NT = 64 * WGM * WGN;
@@ -67,8 +69,6 @@ k_contract_1m(KArgs<Idx> p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int wm = wid % WGM, wn = wid / WGM;
-  int tm, tn;
-  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
   const int64_t M = p.M, N = p.N;
   const int K = int(p.K);
@@ -227,6 +227,31 @@ k_contract_1m(KArgs<Idx> p) {
   }
 }
 
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
+          bool AF, bool BF>
+__global__ void __launch_bounds__(64 * WGM * WGN)
+k_contract_1m(KArgs<Idx> p) {
+  int tm, tn;
+  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+  contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, tm, tn);
+}
+
+// Persistent batched form: workgroup g takes items g, g + G, g + 2G, ... of
+// a work list of (instance, tile) sorted by decreasing work, so the grid
+// (a few workgroups per CU) drains it in LPT order in one launch.
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
+          bool AF, bool BF>
+__global__ void __launch_bounds__(64 * WGM * WGN)
+k_contract_1m_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __restrict__ items,
+                      int nitems) {
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const BatchItem w = items[it];
+    const KArgs<Idx> p = inst[w.inst];
+    contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn);
+    __syncthreads();   // the next tile rewrites LDS
+  }
+}
+
 // One lane per output element: a plain loop over the contracted index.
 // Works for every dtype; used as the cross-check path and for problems the
 // MFMA tiles do not cover (real dtypes).
@@ -318,6 +343,92 @@ const KernelInfo kInfo[K_COUNT] = {
 }  // namespace
 
 hipError_t launch_bf16x3(const GemmProblem& g, int32_t kernel, hipStream_t s);
+int bf16x3_batched_blocks_per_cu(int32_t kernel, bool idx32, bool af, bool bf);
+hipError_t launch_bf16x3_batched(int32_t kernel, bool idx32, bool af, bool bf,
+                                 const void* inst, const BatchItem* items, int nitems,
+                                 int grid, hipStream_t s);
+
+namespace {
+
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN>
+struct Batched1M {
+  using K = void (*)(const KArgs<Idx>*, const BatchItem*, int);
+  static K pick(bool af, bool bf) {
+    if (af && bf) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, true, true>;
+    if (af) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, true, false>;
+    if (bf) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, false, true>;
+    return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, false, false>;
+  }
+  static int per_cu(bool af, bool bf) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pick(af, bf), 64 * WGM * WGN, 0) !=
+        hipSuccess) {
+      (void)hipGetLastError();
+      return 1;
+    }
+    return n > 0 ? n : 1;
+  }
+  static hipError_t launch(bool af, bool bf, const void* inst, const BatchItem* items,
+                           int nitems, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(pick(af, bf), dim3(grid), dim3(64 * WGM * WGN), 0, s,
+                       static_cast<const KArgs<Idx>*>(inst), items, nitems);
+    return hipGetLastError();
+  }
+};
+
+template <typename Idx, typename F>
+auto with_1m_config(int32_t kernel, F&& f) {
+  switch (kernel) {
+    case K_C64_F32_144: return f(Batched1M<float, Idx, 144, 144, 16, 2, 4>());
+    case K_C64_F32_64: return f(Batched1M<float, Idx, 64, 64, 16, 2, 2>());
+    case K_C128_F64_64: return f(Batched1M<double, Idx, 64, 64, 16, 2, 2>());
+    default: return f(Batched1M<double, Idx, 32, 32, 16, 2, 2>());
+  }
+}
+
+}  // namespace
+
+size_t batch_args_size(bool idx32) {
+  return idx32 ? sizeof(KArgs<int32_t>) : sizeof(KArgs<int64_t>);
+}
+
+void batch_make_args(const GemmProblem& g, int32_t kernel, void* out, int* tiles_m,
+                     int* tiles_n) {
+  const KernelInfo& ki = kInfo[(kernel >= 0 && kernel < K_COUNT) ? kernel : 0];
+  if (g.idx32) {
+    const KArgs<int32_t> a = make_args<int32_t>(g, ki.bm, ki.bn);
+    memcpy(out, &a, sizeof(a));
+    *tiles_m = a.tiles_m; *tiles_n = a.tiles_n;
+  } else {
+    const KArgs<int64_t> a = make_args<int64_t>(g, ki.bm, ki.bn);
+    memcpy(out, &a, sizeof(a));
+    *tiles_m = a.tiles_m; *tiles_n = a.tiles_n;
+  }
+}
+
+int batch_blocks_per_cu(int32_t kernel, bool idx32, bool af, bool bk) {
+  const bool bf = !bk;
+  if (kernel == K_C64_BF16X3_144 || kernel == K_C64_BF16X3_64)
+    return bf16x3_batched_blocks_per_cu(kernel, idx32, af, bf);
+  if (idx32) return with_1m_config<int32_t>(kernel, [&](auto c) { return c.per_cu(af, bf); });
+  return with_1m_config<int64_t>(kernel, [&](auto c) { return c.per_cu(af, bf); });
+}
+
+hipError_t launch_contract_batched(int32_t kernel, bool idx32, bool af, bool bk,
+                                   const void* inst, const BatchItem* items, int nitems,
+                                   int grid, hipStream_t s) {
+  const bool bf = !bk;
+  if (nitems == 0) return hipSuccess;
+  if (kernel == K_C64_BF16X3_144 || kernel == K_C64_BF16X3_64)
+    return launch_bf16x3_batched(kernel, idx32, af, bf, inst, items, nitems, grid, s);
+  if (idx32)
+    return with_1m_config<int32_t>(kernel, [&](auto c) {
+      return c.launch(af, bf, inst, items, nitems, grid, s);
+    });
+  return with_1m_config<int64_t>(kernel, [&](auto c) {
+    return c.launch(af, bf, inst, items, nitems, grid, s);
+  });
+}
 
 const KernelInfo& kernel_info(int32_t id) {
   return kInfo[(id >= 0 && id < K_COUNT) ? id : 0];

@@ -83,4 +83,28 @@ const KernelInfo& kernel_info(int32_t id);
 hipError_t launch_contract(const GemmProblem& p, int32_t kernel,
                            hipStream_t stream);
 
+// ---- persistent batched launch --------------------------------------------
+
+// One unit of a batched launch: tile (tm, tn) of instance `inst`.
+struct BatchItem {
+  int32_t inst;
+  int32_t tm;
+  int32_t tn;
+  int32_t work;   // sort key (contracted extent)
+};
+
+// Size in bytes of one instance's kernel arguments for a batched launch.
+size_t batch_args_size(bool idx32);
+// Writes the kernel arguments of problem g for kernel `kernel` into out
+// (batch_args_size bytes) and returns its tile grid (tiles_m x tiles_n).
+void batch_make_args(const GemmProblem& g, int32_t kernel, void* out,
+                     int* tiles_m, int* tiles_n);
+// Resident workgroups per CU of the batched form of `kernel`.
+int batch_blocks_per_cu(int32_t kernel, bool idx32, bool a_mfast, bool b_kfast);
+// Launches the persistent batched kernel over `nitems` work items.
+hipError_t launch_contract_batched(int32_t kernel, bool idx32, bool a_mfast,
+                                   bool b_kfast, const void* inst,
+                                   const BatchItem* items, int nitems, int grid,
+                                   hipStream_t stream);
+
 }  // namespace t1m_hip

@@ -494,6 +494,32 @@ def test_autograd_harness_complex64():
 
 # ---------------------------------------------------------------- batch --
 
+@pytest.mark.parametrize("dtype,prec,tol", [(np.complex64, "native", TOL["fp32"]),
+                                            (np.complex64, "bf16x3", TOL["bf16x3"]),
+                                            (np.complex128, "native", TOL["fp64"]),
+                                            (np.float32, "native", TOL["fp32"])])
+def test_batch_families(dtype, prec, tol):
+    """Persistent batched launch (complex) / per-instance graph (real), with
+    ragged bond dimensions, conj and alpha, against the oracle."""
+    rng = np.random.default_rng(16)
+    la, lb, lc = "asb", "btc", "astc"
+    As, Bs, Cs, C0s, wants = [], [], [], [], []
+    for _ in range(9):
+        dl, db, dr = (int(x) for x in rng.integers(3, 90, 3))
+        A = rand(rng, (dl, 4, db), dtype)
+        B = rand(rng, (db, 4, dr), dtype)
+        C0 = rand(rng, (dl, 4, 4, dr), dtype)
+        alpha = 0.5 - 1j if np.dtype(dtype).kind == "c" else 0.5
+        wants.append(oracle_einsum(A, la, B, lb, C0, lc, alpha=alpha, conj_b=True) - C0)
+        As.append(to_dev(A)); Bs.append(to_dev(B)); Cs.append(to_dev(C0)); C0s.append(C0)
+    batch = t1m_amd.Batch(As, la, Bs, lb, Cs, lc, alpha=alpha, conj_b=True, precision=prec)
+    batch.launch()
+    torch.cuda.synchronize()
+    for c, c0, w in zip(Cs, C0s, wants):
+        assert rel_err(to_host(c) - c0, w) < tol
+    batch.close()
+
+
 def test_batch_matches_single_calls():
     rng = np.random.default_rng(15)
     la, lb, lc = "asb", "btc", "astc"

@@ -196,26 +196,24 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   gb.load_k(utid, 0, K, p.rsB);
   ga.load_data(A);
   gb.load_data(B);
-  if (nst > 1) {
-    ga.load_k(utid, BK, K, p.csA);
-    gb.load_k(utid, BK, K, p.rsB);
-  }
+  ga.load_k(utid, BK, K, p.csA);
+  gb.load_k(utid, BK, K, p.rsB);
   swrite(0, 0);
   __syncthreads();
-  for (int s = 0; s < nst; ++s) {
-    const bool more = s + 1 < nst;
-    if (more) {
-      ga.load_data(A);
-      gb.load_data(B);
-    }
-    if (s + 2 < nst) {
-      ga.load_k(utid, (s + 2) * BK, K, p.csA);
-      gb.load_k(utid, (s + 2) * BK, K, p.rsB);
-    }
+  // The loop body is branch-free (a conditional prefetch made the compiler
+  // wait for every load in flight at the top of each iteration); the last
+  // stage is peeled off so no data is fetched past the end.  The k-offset
+  // prefetch of the stage after the last reads clamped, valid indices.
+  for (int s = 0; s + 1 < nst; ++s) {
+    ga.load_data(A);
+    gb.load_data(B);
+    ga.load_k(utid, (s + 2) * BK, K, p.csA);
+    gb.load_k(utid, (s + 2) * BK, K, p.rsB);
     compute(s & 1);
-    if (more) swrite((s + 1) & 1, (s + 1) * BK);
+    swrite((s + 1) & 1, (s + 1) * BK);
     __syncthreads();
   }
+  compute((nst - 1) & 1);
 
   float* __restrict__ Cr = static_cast<float*>(p.C);
   const bool cplx_alpha = p.alpha_im != 0.0;

@@ -37,6 +37,11 @@
 
 #include <cstring>
 
+// K depth of one LDS stage for the 144 x 144 FP32 tiles.
+#ifndef T1M_BK144
+#define T1M_BK144 16
+#endif
+
 namespace t1m_hip {
 
 namespace {
@@ -124,38 +129,61 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
     xmb[x] = mb < MB ? mb : MB - 1;
   }
 
+  // Fragments of one 4-k chunk, raw (re, im) pairs as read from LDS.
+  struct Frag {
+    C2 a[MB];
+    C2 b[NBW];
+    C2 ax[XB_ALLOC];
+    C2 bx;
+  };
+  auto load_frag = [&](int buf, int kc, Frag& f) {
+    const int q = kc * 4 + g;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) f.a[mb] = sA[buf][q][am0 + mb * 8];
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb) f.b[nb] = sB[buf][q][bn0 + nb * 16];
+    if constexpr (XT) {
+      f.bx = sB[buf][q][bnx];
+#pragma unroll
+      for (int x = 0; x < XB; ++x) f.ax[x] = sA[buf][q][am0 + xmb[x] * 8];
+    }
+  };
+  // 1M expansion in registers, then two MFMAs per tile: B's real parts
+  // against A_r's even real columns, B's imaginary parts against the odd.
+  auto mma = [&](const Frag& f) {
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      const T re = odd ? f.a[mb].y : f.a[mb].x;    // A_r[2i+s][2q]
+      const T im = odd ? f.a[mb].x : -f.a[mb].y;   // A_r[2i+s][2q+1]
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) {
+        acc[mb][nb] = mfma16(f.b[nb].x, re, acc[mb][nb]);
+        acc[mb][nb] = mfma16(f.b[nb].y, im, acc[mb][nb]);
+      }
+    }
+    if constexpr (XT) {
+#pragma unroll
+      for (int x = 0; x < XB; ++x) {
+        const T re = odd ? f.ax[x].y : f.ax[x].x;
+        const T im = odd ? f.ax[x].x : -f.ax[x].y;
+        accx[x] = mfma16(f.bx.x, re, accx[x]);
+        accx[x] = mfma16(f.bx.y, im, accx[x]);
+      }
+    }
+  };
+  // Chunk kc + 1's LDS reads are issued before chunk kc's MFMAs (the
+  // scheduling barriers keep the compiler from sinking them), so a wave
+  // never waits on LDS latency inside a stage.
   auto compute = [&](int buf) {
+    constexpr int KCH = BK / 4;
+    Frag f[2];
+    load_frag(buf, 0, f[0]);
 #pragma unroll
-    for (int kc = 0; kc < BK / 4; ++kc) {
-      const int q = kc * 4 + g;
-      T are[MB], aim[MB];
-#pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-        const C2 a = sA[buf][q][am0 + mb * 8];
-        are[mb] = odd ? a.y : a.x;     // A_r[2i+s][2q]
-        aim[mb] = odd ? a.x : -a.y;    // A_r[2i+s][2q+1]
-      }
-      C2 bf[NBW];
-#pragma unroll
-      for (int nb = 0; nb < NBW; ++nb) bf[nb] = sB[buf][q][bn0 + nb * 16];
-#pragma unroll
-      for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < NBW; ++nb) {
-          acc[mb][nb] = mfma16(bf[nb].x, are[mb], acc[mb][nb]);
-          acc[mb][nb] = mfma16(bf[nb].y, aim[mb], acc[mb][nb]);
-        }
-      if constexpr (XT) {
-        const C2 bx = sB[buf][q][bnx];
-#pragma unroll
-        for (int x = 0; x < XB; ++x) {
-          const C2 a = sA[buf][q][am0 + xmb[x] * 8];
-          const T xr = odd ? a.y : a.x;
-          const T xi = odd ? a.x : -a.y;
-          accx[x] = mfma16(bx.x, xr, accx[x]);
-          accx[x] = mfma16(bx.y, xi, accx[x]);
-        }
-      }
+    for (int kc = 0; kc < KCH; ++kc) {
+      if (kc + 1 < KCH) load_frag(buf, kc + 1, f[(kc + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(f[kc & 1]);
+      __builtin_amdgcn_sched_barrier(0);
     }
   };
 
@@ -167,26 +195,24 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   gb.load_k(utid, 0, K, p.rsB);
   ga.load_data(A);
   gb.load_data(B);
-  if (nst > 1) {
-    ga.load_k(utid, BK, K, p.csA);
-    gb.load_k(utid, BK, K, p.rsB);
-  }
+  ga.load_k(utid, BK, K, p.csA);
+  gb.load_k(utid, BK, K, p.rsB);
   swrite(0, 0);
   __syncthreads();
-  for (int s = 0; s < nst; ++s) {
-    const bool more = s + 1 < nst;
-    if (more) {
-      ga.load_data(A);
-      gb.load_data(B);
-    }
-    if (s + 2 < nst) {
-      ga.load_k(utid, (s + 2) * BK, K, p.csA);
-      gb.load_k(utid, (s + 2) * BK, K, p.rsB);
-    }
+  // The loop body is branch-free (a conditional prefetch made the compiler
+  // wait for every load in flight at the top of each iteration); the last
+  // stage is peeled off so no data is fetched past the end.  The k-offset
+  // prefetch of the stage after the last reads clamped, valid indices.
+  for (int s = 0; s + 1 < nst; ++s) {
+    ga.load_data(A);
+    gb.load_data(B);
+    ga.load_k(utid, (s + 2) * BK, K, p.csA);
+    gb.load_k(utid, (s + 2) * BK, K, p.rsB);
     compute(s & 1);
-    if (more) swrite((s + 1) & 1, (s + 1) * BK);
+    swrite((s + 1) & 1, (s + 1) * BK);
     __syncthreads();
   }
+  compute((nst - 1) & 1);
 
   // ---- epilogue: C[rsC[i] + csC[j]] += alpha * acc -----------------------
   T* __restrict__ Cr = static_cast<T*>(p.C);
@@ -322,7 +348,7 @@ hipError_t launch_1m(const GemmProblem& g, hipStream_t s) {
 template <typename Idx>
 hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
   switch (kernel) {
-    case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, 16, 2, 4>(g, s);
+    case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, T1M_BK144, 2, 4>(g, s);
     case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 16, 2, 2>(g, s);
     case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16, 2, 2>(g, s);
     case K_C128_F64_32: return launch_1m<double, Idx, 32, 32, 16, 2, 2>(g, s);
@@ -379,7 +405,7 @@ struct Batched1M {
 template <typename Idx, typename F>
 auto with_1m_config(int32_t kernel, F&& f) {
   switch (kernel) {
-    case K_C64_F32_144: return f(Batched1M<float, Idx, 144, 144, 16, 2, 4>());
+    case K_C64_F32_144: return f(Batched1M<float, Idx, 144, 144, T1M_BK144, 2, 4>());
     case K_C64_F32_64: return f(Batched1M<float, Idx, 64, 64, 16, 2, 2>());
     case K_C128_F64_64: return f(Batched1M<double, Idx, 64, 64, 16, 2, 2>());
     default: return f(Batched1M<double, Idx, 32, 32, 16, 2, 2>());

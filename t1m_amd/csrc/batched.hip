@@ -9,8 +9,8 @@
 // work of a tile is proportional to K: longest-processing-time-first order).
 // The graph holds a single persistent launch of a few workgroups per CU that
 // walk that list with a stride of the grid size.  When the instances cannot
-// share one kernel (real dtypes, mixed gather directions or offset widths),
-// the graph holds one launch per instance instead.
+// share one kernel (mixed gather directions or offset widths, or the simple
+// kernel), the graph holds one launch per instance instead.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,7 +59,7 @@ int cu_count_batch() {
 int batch_kernel(int dtype, int precision) {
   if (const char* f = getenv("T1M_FORCE_KERNEL")) {
     const int k = atoi(f);
-    if (k > 0 && k < K_COUNT) return k;
+    if (kernel_fits(k, dtype, precision)) return k;
   }
   if (precision == T1M_PRECISION_SIMPLE) return K_SIMPLE;
   switch (dtype) {
@@ -67,6 +67,10 @@ int batch_kernel(int dtype, int precision) {
       return precision == T1M_PRECISION_BF16X3 ? K_C64_BF16X3_64 : K_C64_F32_64;
     case T1M_COMPLEX128:
       return K_C128_F64_32;
+    case T1M_FLOAT32:
+      return K_F32_64;
+    case T1M_FLOAT64:
+      return K_F64_32;
     default:
       return K_SIMPLE;
   }

@@ -364,11 +364,20 @@ const KernelInfo kInfo[K_COUNT] = {
     {"c128_f64mfma_32x32", 32, 32},
     {"c64_bf16x3_144x144", 144, 144},
     {"c64_bf16x3_64x64", 64, 64},
+    {"f32mfma_128x128", 128, 128},
+    {"f32mfma_64x64", 64, 64},
+    {"f64mfma_64x64", 64, 64},
+    {"f64mfma_32x32", 32, 32},
 };
 
 }  // namespace
 
 hipError_t launch_bf16x3(const GemmProblem& g, int32_t kernel, hipStream_t s);
+bool is_real_kernel(int32_t kernel);
+hipError_t launch_real(const GemmProblem& g, int32_t kernel, hipStream_t s);
+int real_batched_blocks_per_cu(int32_t kernel, bool idx32, bool af, bool bf);
+hipError_t launch_real_batched(int32_t kernel, bool idx32, bool af, bool bf, const void* inst,
+                               const BatchItem* items, int nitems, int grid, hipStream_t s);
 int bf16x3_batched_blocks_per_cu(int32_t kernel, bool idx32, bool af, bool bf);
 hipError_t launch_bf16x3_batched(int32_t kernel, bool idx32, bool af, bool bf,
                                  const void* inst, const BatchItem* items, int nitems,
@@ -436,6 +445,7 @@ int batch_blocks_per_cu(int32_t kernel, bool idx32, bool af, bool bk) {
   const bool bf = !bk;
   if (kernel == K_C64_BF16X3_144 || kernel == K_C64_BF16X3_64)
     return bf16x3_batched_blocks_per_cu(kernel, idx32, af, bf);
+  if (is_real_kernel(kernel)) return real_batched_blocks_per_cu(kernel, idx32, af, bf);
   if (idx32) return with_1m_config<int32_t>(kernel, [&](auto c) { return c.per_cu(af, bf); });
   return with_1m_config<int64_t>(kernel, [&](auto c) { return c.per_cu(af, bf); });
 }
@@ -447,6 +457,8 @@ hipError_t launch_contract_batched(int32_t kernel, bool idx32, bool af, bool bk,
   if (nitems == 0) return hipSuccess;
   if (kernel == K_C64_BF16X3_144 || kernel == K_C64_BF16X3_64)
     return launch_bf16x3_batched(kernel, idx32, af, bf, inst, items, nitems, grid, s);
+  if (is_real_kernel(kernel))
+    return launch_real_batched(kernel, idx32, af, bf, inst, items, nitems, grid, s);
   if (idx32)
     return with_1m_config<int32_t>(kernel, [&](auto c) {
       return c.launch(af, bf, inst, items, nitems, grid, s);
@@ -465,6 +477,7 @@ hipError_t launch_contract(const GemmProblem& g, int32_t kernel,
   if (g.M == 0 || g.N == 0) return hipSuccess;
   if (kernel == K_C64_BF16X3_144 || kernel == K_C64_BF16X3_64)
     return launch_bf16x3(g, kernel, s);
+  if (is_real_kernel(kernel)) return launch_real(g, kernel, s);
   return g.idx32 ? dispatch<int32_t>(g, kernel, s) : dispatch<int64_t>(g, kernel, s);
 }
 

@@ -1,0 +1,278 @@
+// Real contraction C += alpha * A . B (t1m::contract for float / double,
+// include/t1m/t1m.h:560-568; real branch of the driver t1m.h:466-507, whose
+// micro-kernel call is bli_sgemm_ukernel / bli_dgemm_ukernel,
+// bli/mappings.h:91-99) on gfx950 matrix cores.
+//
+// Same structure as the complex kernels (contract_mfma.hip): one workgroup
+// per BM x BN tile of C, strided gathers into double-buffered LDS, MFMA
+// fragments read one 4-k chunk ahead, scatter-accumulate epilogue through
+// the scatter vectors.  Without 1M the MFMA is the plain real product: in
+// the transposed form the MFMA rows are 16 columns j of C and its columns
+// are 16 rows i, so lane l holds C[i = l % 16][.] and 16 lanes write 16
+// consecutive rows.  v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64, one MFMA
+// per 4 k per 16 x 16 tile.
+#include "internal.h"
+#include "kernel_common.h"
+
+#include <cstring>
+
+namespace t1m_hip {
+
+namespace {
+
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
+          bool AF, bool BF>
+__device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, int tn) {
+  using Acc = typename Vec<T>::acc;
+  constexpr int NT = 64 * WGM * WGN;
+  constexpr int WTM = BM / WGM;          // rows per wave
+  constexpr int MB = WTM / 16;           // 16-row blocks per wave
+  constexpr int NBT = BN / 16;
+  constexpr int NBW = NBT / WGN;
+  constexpr bool XT = NBT % WGN != 0;
+  constexpr int XB = XT ? (MB + WGN - 1) / WGN : 0;
+  constexpr int XB_ALLOC = XT ? XB : 1;
+  constexpr int PAD = sizeof(T) == 4 ? 16 : 8;
+  constexpr int LDA = BM + PAD, LDB = BN + PAD;
+  static_assert(WTM % 16 == 0 && BN % 16 == 0, "tile shape");
+  static_assert(NBT - NBW * WGN <= 1, "at most one shared column block");
+
+  __shared__ T sA[2][BK][LDA];
+  __shared__ T sB[2][BK][LDB];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int wm = wid % WGM, wn = wid / WGM;
+  const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
+  const int64_t M = p.M, N = p.N;
+  const int K = int(p.K);
+  const T* __restrict__ A = static_cast<const T*>(p.A);
+  const T* __restrict__ B = static_cast<const T*>(p.B);
+
+  const unsigned utid = unsigned(tid);
+  using GA = TileGather<T, Idx, BM, BK, NT, AF>;
+  using GB = TileGather<T, Idx, BN, BK, NT, BF>;
+  GA ga;
+  GB gb;
+  ga.init(utid, i0, M, p.rsA);
+  gb.init(utid, j0, N, p.csB);
+
+  auto swrite = [&](int buf, int p0) {
+    ga.finish_real(utid, p0, K);
+    gb.finish_real(utid, p0, K);
+#pragma unroll
+    for (int u = 0; u < GA::E; ++u) {
+      const unsigned e = GA::elem(utid, u);
+      if (GA::valid(utid, u)) sA[buf][GA::kc(e)][GA::xc(e)] = ga.val[u];
+    }
+#pragma unroll
+    for (int u = 0; u < GB::E; ++u) {
+      const unsigned e = GB::elem(utid, u);
+      if (GB::valid(utid, u)) sB[buf][GB::kc(e)][GB::xc(e)] = gb.val[u];
+    }
+  };
+
+  Acc acc[MB][NBW];
+  Acc accx[XB_ALLOC];
+#pragma unroll
+  for (int a = 0; a < MB; ++a)
+#pragma unroll
+    for (int b = 0; b < NBW; ++b) acc[a][b] = Acc{0, 0, 0, 0};
+#pragma unroll
+  for (int x = 0; x < XB_ALLOC; ++x) accx[x] = Acc{0, 0, 0, 0};
+
+  const int g = lane >> 4;
+  const int l16 = lane & 15;
+  const int am0 = wm * WTM + l16;
+  const int bn0 = wn * NBW * 16 + l16;
+  const int bnx = NBW * WGN * 16 + l16;
+  int xmb[XB_ALLOC];
+#pragma unroll
+  for (int x = 0; x < XB_ALLOC; ++x) {
+    const int mb = wn * XB + x;
+    xmb[x] = mb < MB ? mb : MB - 1;
+  }
+
+  struct Frag {
+    T a[MB];
+    T b[NBW];
+    T ax[XB_ALLOC];
+    T bx;
+  };
+  auto load_frag = [&](int buf, int kc, Frag& f) {
+    const int q = kc * 4 + g;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) f.a[mb] = sA[buf][q][am0 + mb * 16];
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb) f.b[nb] = sB[buf][q][bn0 + nb * 16];
+    if constexpr (XT) {
+      f.bx = sB[buf][q][bnx];
+#pragma unroll
+      for (int x = 0; x < XB; ++x) f.ax[x] = sA[buf][q][am0 + xmb[x] * 16];
+    }
+  };
+  auto mma = [&](const Frag& f) {
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) acc[mb][nb] = mfma16(f.b[nb], f.a[mb], acc[mb][nb]);
+    if constexpr (XT) {
+#pragma unroll
+      for (int x = 0; x < XB; ++x) accx[x] = mfma16(f.bx, f.ax[x], accx[x]);
+    }
+  };
+  auto compute = [&](int buf) {
+    constexpr int KCH = BK / 4;
+    Frag f[2];
+    load_frag(buf, 0, f[0]);
+#pragma unroll
+    for (int kc = 0; kc < KCH; ++kc) {
+      if (kc + 1 < KCH) load_frag(buf, kc + 1, f[(kc + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(f[kc & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+
+  const int nst = (K + BK - 1) / BK;
+  ga.load_k(utid, 0, K, p.csA);
+  gb.load_k(utid, 0, K, p.rsB);
+  ga.load_data(A);
+  gb.load_data(B);
+  ga.load_k(utid, BK, K, p.csA);
+  gb.load_k(utid, BK, K, p.rsB);
+  swrite(0, 0);
+  __syncthreads();
+  for (int s = 0; s + 1 < nst; ++s) {
+    ga.load_data(A);
+    gb.load_data(B);
+    ga.load_k(utid, (s + 2) * BK, K, p.csA);
+    gb.load_k(utid, (s + 2) * BK, K, p.rsB);
+    compute(s & 1);
+    swrite((s + 1) & 1, (s + 1) * BK);
+    __syncthreads();
+  }
+  compute((nst - 1) & 1);
+
+  T* __restrict__ C = static_cast<T*>(p.C);
+  const T alpha = T(p.alpha_re);
+  auto store = [&](const Acc& v, int64_t i, int64_t jb) {
+    const bool iok = i < M;
+    const Idx rc = iok ? p.rsC[i] : Idx(0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      // MFMA output row of (lane, r): 4(lane/16) + r (f32), lane/16 + 4r (f64).
+      const int64_t j = jb + (sizeof(T) == 4 ? 4 * g + r : g + 4 * r);
+      if (iok && j < N) C[int64_t(rc + p.csC[j])] += alpha * v[r];
+    }
+  };
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb)
+      store(acc[mb][nb], i0 + am0 + mb * 16, j0 + wn * NBW * 16 + nb * 16);
+  if constexpr (XT) {
+#pragma unroll
+    for (int x = 0; x < XB; ++x)
+      if (wn * XB + x < MB) store(accx[x], i0 + am0 + (wn * XB + x) * 16, j0 + NBW * WGN * 16);
+  }
+}
+
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
+__global__ void __launch_bounds__(64 * WGM * WGN) k_contract_real(KArgs<Idx> p) {
+  int tm, tn;
+  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+  contract_tile_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, tm, tn);
+}
+
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
+__global__ void __launch_bounds__(64 * WGM * WGN)
+k_contract_real_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __restrict__ items,
+                        int nitems) {
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const BatchItem w = items[it];
+    const KArgs<Idx> p = inst[w.inst];
+    contract_tile_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn);
+    __syncthreads();
+  }
+}
+
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN>
+struct RealCfg {
+  static constexpr int bm = BM, bn = BN;
+  using KB = void (*)(const KArgs<Idx>*, const BatchItem*, int);
+  template <bool AF, bool BF>
+  static hipError_t launch_one(const KArgs<Idx>& a, hipStream_t s) {
+    const dim3 grid(unsigned(a.tiles_m * a.tiles_n)), block(64 * WGM * WGN);
+    hipLaunchKernelGGL((k_contract_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>), grid, block, 0, s, a);
+    return hipGetLastError();
+  }
+  static hipError_t launch(const GemmProblem& g, hipStream_t s) {
+    const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
+    const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
+    if (af && bf) return launch_one<true, true>(a, s);
+    if (af) return launch_one<true, false>(a, s);
+    if (bf) return launch_one<false, true>(a, s);
+    return launch_one<false, false>(a, s);
+  }
+  static KB pick(bool af, bool bf) {
+    if (af && bf) return k_contract_real_batched<T, Idx, BM, BN, BK, WGM, WGN, true, true>;
+    if (af) return k_contract_real_batched<T, Idx, BM, BN, BK, WGM, WGN, true, false>;
+    if (bf) return k_contract_real_batched<T, Idx, BM, BN, BK, WGM, WGN, false, true>;
+    return k_contract_real_batched<T, Idx, BM, BN, BK, WGM, WGN, false, false>;
+  }
+  static int per_cu(bool af, bool bf) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pick(af, bf), 64 * WGM * WGN, 0) !=
+        hipSuccess) {
+      (void)hipGetLastError();
+      return 1;
+    }
+    return n > 0 ? n : 1;
+  }
+  static hipError_t launch_batched(bool af, bool bf, const void* inst, const BatchItem* items,
+                                   int nitems, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(pick(af, bf), dim3(grid), dim3(64 * WGM * WGN), 0, s,
+                       static_cast<const KArgs<Idx>*>(inst), items, nitems);
+    return hipGetLastError();
+  }
+};
+
+template <typename Idx, typename F>
+auto with_real_config(int32_t kernel, F&& f) {
+  switch (kernel) {
+    case K_F32_128: return f(RealCfg<float, Idx, 128, 128, 16, 2, 2>());
+    case K_F32_64: return f(RealCfg<float, Idx, 64, 64, 16, 2, 2>());
+    case K_F64_64: return f(RealCfg<double, Idx, 64, 64, 16, 2, 2>());
+    default: return f(RealCfg<double, Idx, 32, 32, 16, 2, 2>());
+  }
+}
+
+}  // namespace
+
+bool is_real_kernel(int32_t kernel) {
+  return kernel == K_F32_128 || kernel == K_F32_64 || kernel == K_F64_64 || kernel == K_F64_32;
+}
+
+hipError_t launch_real(const GemmProblem& g, int32_t kernel, hipStream_t s) {
+  if (g.idx32) return with_real_config<int32_t>(kernel, [&](auto c) { return c.launch(g, s); });
+  return with_real_config<int64_t>(kernel, [&](auto c) { return c.launch(g, s); });
+}
+
+int real_batched_blocks_per_cu(int32_t kernel, bool idx32, bool af, bool bf) {
+  if (idx32) return with_real_config<int32_t>(kernel, [&](auto c) { return c.per_cu(af, bf); });
+  return with_real_config<int64_t>(kernel, [&](auto c) { return c.per_cu(af, bf); });
+}
+
+hipError_t launch_real_batched(int32_t kernel, bool idx32, bool af, bool bf, const void* inst,
+                               const BatchItem* items, int nitems, int grid, hipStream_t s) {
+  if (idx32)
+    return with_real_config<int32_t>(kernel, [&](auto c) {
+      return c.launch_batched(af, bf, inst, items, nitems, grid, s);
+    });
+  return with_real_config<int64_t>(kernel, [&](auto c) {
+    return c.launch_batched(af, bf, inst, items, nitems, grid, s);
+  });
+}
+
+}  // namespace t1m_hip

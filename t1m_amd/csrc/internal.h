@@ -71,6 +71,10 @@ enum KernelId : int32_t {
   K_C128_F64_32 = 4,   // complex<double>, 1M on FP64 MFMA, 32x32 tiles
   K_C64_BF16X3_144 = 5,// complex<float>, 1M BF16x3 MFMA, 144x144 tiles
   K_C64_BF16X3_64 = 6, // complex<float>, 1M BF16x3 MFMA, 64x64 tiles
+  K_F32_128 = 7,       // float, FP32 MFMA, 128x128 tiles
+  K_F32_64 = 8,        // float, FP32 MFMA, 64x64 tiles
+  K_F64_64 = 9,        // double, FP64 MFMA, 64x64 tiles
+  K_F64_32 = 10,       // double, FP64 MFMA, 32x32 tiles
   K_COUNT
 };
 

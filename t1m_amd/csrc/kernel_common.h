@@ -117,6 +117,15 @@ struct TileGather {
 #pragma unroll
     for (int u = 0; u < E; ++u) val[u] = base[xoff[u] + koff[u]];
   }
+  // Zero out-of-range entries (real data).
+  __device__ void finish_real(unsigned tid, int p0, int K) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int kg = p0 + kc(elem(tid, u));
+      const bool ok = ((xok >> u) & 1u) && kg < K;
+      val[u] = ok ? val[u] : T2(0);
+    }
+  }
   // Zero out-of-range entries and conjugate if asked (sign = -1).
   template <typename T>
   __device__ void finish(unsigned tid, int p0, int K, T sign) {

@@ -87,6 +87,8 @@ void plan_cache_clear();
 bool is_device_pointer(const void* ptr);
 // Kernel ids worth timing for a plan's dtype and precision.
 std::vector<int> kernel_candidates(int dtype, int precision);
+// Whether kernel id `kernel` computes this dtype at this precision.
+bool kernel_fits(int kernel, int dtype, int precision);
 // Times the candidates on a scratch copy of C's extent and stores the
 // fastest in the plan (and in the tuning cache file).
 int autotune_plan(Plan& plan, const GemmProblem& g, hipStream_t stream);

@@ -202,9 +202,20 @@ std::vector<int> kernel_candidates(int dtype, int precision) {
       return {K_C64_F32_144, K_C64_F32_64};
     case T1M_COMPLEX128:
       return {K_C128_F64_64, K_C128_F64_32};
+    case T1M_FLOAT32:
+      return {K_F32_128, K_F32_64};
+    case T1M_FLOAT64:
+      return {K_F64_64, K_F64_32};
     default:
       return {K_SIMPLE};
   }
+}
+
+bool kernel_fits(int kernel, int dtype, int precision) {
+  if (kernel == K_SIMPLE) return true;
+  for (int k : kernel_candidates(dtype, precision))
+    if (k == kernel) return true;
+  return false;
 }
 
 // Static rule, used when no tuning result is known: the large tiles (one
@@ -212,12 +223,12 @@ std::vector<int> kernel_candidates(int dtype, int precision) {
 // otherwise the small tiles (several workgroups per CU).  Calibrated on the
 // 48^4 and 256x4x256 workloads.
 int choose_kernel(const Problem& P, int precision) {
-  if (const char* f = getenv("T1M_FORCE_KERNEL")) {
-    const int k = atoi(f);
-    if (k >= 0 && k < K_COUNT) return k;
+  if (P.K >= (int64_t(1) << 31)) return K_SIMPLE;     // MFMA kernels use int k
+  if (const char* f = getenv("T1M_FORCE_KERNEL")) {   // tests / tuning; ignored
+    const int k = atoi(f);                            // when it does not fit
+    if (kernel_fits(k, P.dtype, precision)) return k;
   }
   if (precision == T1M_PRECISION_SIMPLE) return K_SIMPLE;
-  if (P.K >= (int64_t(1) << 31)) return K_SIMPLE;     // MFMA kernels use int k
   auto tiles = [&](int bm, int bn) {
     return double((P.M + bm - 1) / bm) * double((P.N + bn - 1) / bn);
   };
@@ -229,6 +240,10 @@ int choose_kernel(const Problem& P, int precision) {
     }
     case T1M_COMPLEX128:
       return tiles(64, 64) >= 2.0 * cu_count() ? K_C128_F64_64 : K_C128_F64_32;
+    case T1M_FLOAT32:
+      return tiles(128, 128) >= cu_count() ? K_F32_128 : K_F32_64;
+    case T1M_FLOAT64:
+      return tiles(64, 64) >= 2.0 * cu_count() ? K_F64_64 : K_F64_32;
     default:
       return K_SIMPLE;
   }

@@ -232,17 +232,26 @@ def test_ragged_all_kernels(case, idx64):
             assert err < TOL[tolkey], (kern, err)
 
 
+REAL_KERNELS = {np.float32: ["f32mfma_128x128", "f32mfma_64x64", "simple"],
+                np.float64: ["f64mfma_64x64", "f64mfma_32x32", "simple"]}
+
+
+@pytest.mark.parametrize("idx64", [False, True])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_real_contractions(dtype):
-    """Real contract(alpha, ...) (t1m.h:560-568): C += alpha * A . B."""
+def test_real_contractions(dtype, idx64):
+    """Real contract(alpha, ...) (t1m.h:560-568): C += alpha * A . B, every
+    real kernel on ragged shapes."""
     rng = np.random.default_rng(5)
-    for (la, lb, lc), ext in RAGGED[:3]:
+    for (la, lb, lc), ext in RAGGED:
         A = rand(rng, [ext[x] for x in la], dtype)
         B = rand(rng, [ext[x] for x in lb], dtype)
         C0 = rand(rng, [ext[x] for x in lc], dtype)
         want = oracle_einsum(A, la, B, lb, C0, lc, alpha=0.75)
-        got = run(A, la, B, lb, C0, lc, alpha=0.75)
-        assert rel_err(got - C0, want - C0) < (2e-6 if dtype == np.float32 else 1e-13)
+        for kern in REAL_KERNELS[dtype]:
+            with forced(kern, idx64):
+                got = run(A, la, B, lb, C0, lc, alpha=0.75)
+            err = rel_err(got - C0, want - C0)
+            assert err < (2e-6 if dtype == np.float32 else 1e-13), (la, lb, lc, kern, err)
 
 
 def test_conj_alpha_and_accumulate():
@@ -497,7 +506,8 @@ def test_autograd_harness_complex64():
 @pytest.mark.parametrize("dtype,prec,tol", [(np.complex64, "native", TOL["fp32"]),
                                             (np.complex64, "bf16x3", TOL["bf16x3"]),
                                             (np.complex128, "native", TOL["fp64"]),
-                                            (np.float32, "native", TOL["fp32"])])
+                                            (np.float32, "native", TOL["fp32"]),
+                                            (np.float64, "native", TOL["fp64"])])
 def test_batch_families(dtype, prec, tol):
     """Persistent batched launch (complex) / per-instance graph (real), with
     ragged bond dimensions, conj and alpha, against the oracle."""

@@ -35,6 +35,7 @@ struct context {
   int precision = T1M_PRECISION_NATIVE;       // t1m_precision
   bool synchronize = true;                    // result visible on return
   bool autotune = false;                      // time tile shapes on first use
+  bool deterministic = false;                 // never split K (no atomics)
 };
 
 namespace detail {
@@ -92,6 +93,7 @@ void run(double alpha, const tensor<T, na>& a, const std::string& la,
     o.precision = ctx->precision;
     o.synchronize = ctx->synchronize ? 1 : 0;
     o.autotune = ctx->autotune ? 1 : 0;
+    o.deterministic = ctx->deterministic ? 1 : 0;
   }
   raise(t1m_contract(&A.t, la.c_str(), &B.t, lb.c_str(), &C.t, lc.c_str(), &o));
 }

@@ -75,10 +75,16 @@ typedef struct t1m_options {
   int32_t autotune;         /* nonzero: the first call of a new shape times
                                the candidate tile shapes (on a scratch C) and
                                keeps the fastest; see t1m_autotune_cache_path */
+  int32_t deterministic;    /* nonzero: never split K across workgroups, so
+                               results are bitwise reproducible; by default a
+                               contraction whose C tiles cannot fill the GPU
+                               splits K and adds the partial sums into C with
+                               atomics (order of those additions varies) */
 } t1m_options;
 
 /* alpha = 1, NATIVE, no conjugation, synchronize = 1, null stream,
- * autotune = 0 (tile shape from the tuning cache, else a static rule). */
+ * autotune = 0 (tile shape from the tuning cache, else a static rule),
+ * deterministic = 0. */
 void t1m_options_init(t1m_options* opt);
 
 /*

@@ -37,7 +37,8 @@ class Options(ctypes.Structure):
     _fields_ = [("alpha_re", ctypes.c_double), ("alpha_im", ctypes.c_double),
                 ("precision", ctypes.c_int32), ("conj_a", ctypes.c_int32),
                 ("conj_b", ctypes.c_int32), ("synchronize", ctypes.c_int32),
-                ("stream", ctypes.c_void_p), ("autotune", ctypes.c_int32)]
+                ("stream", ctypes.c_void_p), ("autotune", ctypes.c_int32),
+                ("deterministic", ctypes.c_int32)]
 
 
 class View(ctypes.Structure):
@@ -139,7 +140,8 @@ class _Desc:
 
 
 def _options(alpha=1.0, precision="native", conj_a=False, conj_b=False,
-             stream=None, synchronize=True, autotune=False) -> Options:
+             stream=None, synchronize=True, autotune=False,
+             deterministic=False) -> Options:
     o = Options()
     load().t1m_options_init(ctypes.byref(o))
     al = complex(alpha)
@@ -149,6 +151,7 @@ def _options(alpha=1.0, precision="native", conj_a=False, conj_b=False,
     o.synchronize = int(bool(synchronize))
     o.stream = _stream_handle(stream)
     o.autotune = int(bool(autotune))
+    o.deterministic = int(bool(deterministic))
     return o
 
 
@@ -168,14 +171,17 @@ def _stream_handle(stream) -> Optional[int]:
 
 def contract(a, la: str, b, lb: str, c, lc: str, alpha=1.0, precision="native",
              conj_a: bool = False, conj_b: bool = False, stream=None,
-             synchronize: bool = True, autotune: bool = False):
+             synchronize: bool = True, autotune: bool = False,
+             deterministic: bool = False):
     """C += alpha * A . B over labels (t1m::contract, t1m.h:560-577).
 
     a, b, c are torch tensors of one dtype (float32/64, complex64/128) with any
     strides; device tensors run without copies.  autotune=True times the
-    candidate tile shapes the first time a shape is seen.  Returns c."""
+    candidate tile shapes the first time a shape is seen; deterministic=True
+    forbids splitting K (bitwise-reproducible results).  Returns c."""
     da, db, dc = _Desc(a), _Desc(b), _Desc(c)
-    o = _options(alpha, precision, conj_a, conj_b, stream, synchronize, autotune)
+    o = _options(alpha, precision, conj_a, conj_b, stream, synchronize, autotune,
+                 deterministic)
     _check(load().t1m_contract(ctypes.byref(da.desc), la.encode(), ctypes.byref(db.desc),
                                lb.encode(), ctypes.byref(dc.desc), lc.encode(),
                                ctypes.byref(o)))

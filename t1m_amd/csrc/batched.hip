@@ -159,6 +159,8 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
     g.conj_a = cplx && o.conj_a; g.conj_b = cplx && o.conj_b;
     g.a_mfast = plan->a_mfast; g.b_kfast = plan->b_kfast;
     g.idx32 = plan->idx32; g.dtype = P.dtype;
+    g.split_k = 1;   // instances run side by side; no atomics needed
+    g.cus = 0;
     bt->plans.push_back(plan);
     bt->problems.push_back(g);
   }

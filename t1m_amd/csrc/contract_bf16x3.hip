@@ -52,7 +52,7 @@ __device__ inline f32x4 mfma_bf16(const u32x4& a, const u32x4& b, f32x4 c) {
 }
 
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
-__device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm, int tn) {
+__device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm, int tn, int kbeg, int kend) {
   constexpr int NT = 64 * WGM * WGN;
   constexpr int WTM = BM / WGM;
   constexpr int MB = WTM / 8;
@@ -76,7 +76,7 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   const int wm = wid % WGM, wn = wid / WGM;
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
   const int64_t M = p.M, N = p.N;
-  const int K = int(p.K);
+  const int K = kend;     // contracted range [kbeg, kend) of this workgroup
   const float2* __restrict__ A = static_cast<const float2*>(p.A);
   const float2* __restrict__ B = static_cast<const float2*>(p.B);
 
@@ -191,14 +191,14 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
     }
   };
 
-  const int nst = (K + BK - 1) / BK;
-  ga.load_k(utid, 0, K, p.csA);
-  gb.load_k(utid, 0, K, p.rsB);
+  const int nst = (K - kbeg + BK - 1) / BK;
+  ga.load_k(utid, kbeg, K, p.csA);
+  gb.load_k(utid, kbeg, K, p.rsB);
   ga.load_data(A);
   gb.load_data(B);
-  ga.load_k(utid, BK, K, p.csA);
-  gb.load_k(utid, BK, K, p.rsB);
-  swrite(0, 0);
+  ga.load_k(utid, kbeg + BK, K, p.csA);
+  gb.load_k(utid, kbeg + BK, K, p.rsB);
+  swrite(0, kbeg);
   __syncthreads();
   // The loop body is branch-free (a conditional prefetch made the compiler
   // wait for every load in flight at the top of each iteration); the last
@@ -207,10 +207,10 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   for (int s = 0; s + 1 < nst; ++s) {
     ga.load_data(A);
     gb.load_data(B);
-    ga.load_k(utid, (s + 2) * BK, K, p.csA);
-    gb.load_k(utid, (s + 2) * BK, K, p.rsB);
+    ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
+    gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
     compute(s & 1);
-    swrite((s + 1) & 1, (s + 1) * BK);
+    swrite((s + 1) & 1, kbeg + (s + 1) * BK);
     __syncthreads();
   }
   compute((nst - 1) & 1);
@@ -231,7 +231,8 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
         val = ar * val;
       }
       const int64_t j = jb + 4 * g + r;
-      if (iok && j < N) Cr[2 * int64_t(rc + p.csC[j]) + (odd ? 1 : 0)] += val;
+      if (iok && j < N)
+        accumulate(Cr + 2 * int64_t(rc + p.csC[j]) + (odd ? 1 : 0), val, p.ksplit > 1);
     }
   };
 #pragma unroll
@@ -250,9 +251,9 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
 __global__ void __launch_bounds__(64 * WGM * WGN)
 k_contract_bf16x3(KArgs<Idx> p) {
-  int tm, tn;
-  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
-  contract_tile_bf16x3<Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, tm, tn);
+  int tm, tn, kbeg, kend;
+  tile_and_krange(p, tm, tn, kbeg, kend);
+  contract_tile_bf16x3<Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, tm, tn, kbeg, kend);
 }
 
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
@@ -262,7 +263,7 @@ k_contract_bf16x3_batched(const KArgs<Idx>* __restrict__ inst,
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const BatchItem w = items[it];
     const KArgs<Idx> p = inst[w.inst];
-    contract_tile_bf16x3<Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn);
+    contract_tile_bf16x3<Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn, 0, int(p.K));
     __syncthreads();
   }
 }
@@ -270,7 +271,7 @@ k_contract_bf16x3_batched(const KArgs<Idx>* __restrict__ inst,
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN>
 hipError_t launch_t(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
-  const dim3 grid(unsigned(a.tiles_m * a.tiles_n)), block(64 * WGM * WGN);
+  const dim3 grid(unsigned(a.tiles_m * a.tiles_n * a.ksplit)), block(64 * WGM * WGN);
   const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
   if (af && bf)
     hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, a);

@@ -49,7 +49,7 @@ namespace {
 // One BM x BN tile (tm, tn) of one contraction, whole K range.
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
           bool AF, bool BF>
-__device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, int tn) {
+__device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, int tn, int kbeg, int kend) {
   using C2 = typename Vec<T>::c2;
   using Acc = typename Vec<T>::acc;
   constexpr int NT = 64 * WGM * WGN;
@@ -76,7 +76,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   const int wm = wid % WGM, wn = wid / WGM;
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
   const int64_t M = p.M, N = p.N;
-  const int K = int(p.K);
+  const int K = kend;     // contracted range [kbeg, kend) of this workgroup
 
   const C2* __restrict__ A = static_cast<const C2*>(p.A);
   const C2* __restrict__ B = static_cast<const C2*>(p.B);
@@ -190,14 +190,14 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   // ---- main loop: double-buffered LDS, one barrier per stage ------------
   // Iteration s multiplies stage s from LDS while the data loads of stage
   // s+1 and the k-offset loads of stage s+2 are in flight.
-  const int nst = (K + BK - 1) / BK;
-  ga.load_k(utid, 0, K, p.csA);
-  gb.load_k(utid, 0, K, p.rsB);
+  const int nst = (K - kbeg + BK - 1) / BK;
+  ga.load_k(utid, kbeg, K, p.csA);
+  gb.load_k(utid, kbeg, K, p.rsB);
   ga.load_data(A);
   gb.load_data(B);
-  ga.load_k(utid, BK, K, p.csA);
-  gb.load_k(utid, BK, K, p.rsB);
-  swrite(0, 0);
+  ga.load_k(utid, kbeg + BK, K, p.csA);
+  gb.load_k(utid, kbeg + BK, K, p.rsB);
+  swrite(0, kbeg);
   __syncthreads();
   // The loop body is branch-free (a conditional prefetch made the compiler
   // wait for every load in flight at the top of each iteration); the last
@@ -206,10 +206,10 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   for (int s = 0; s + 1 < nst; ++s) {
     ga.load_data(A);
     gb.load_data(B);
-    ga.load_k(utid, (s + 2) * BK, K, p.csA);
-    gb.load_k(utid, (s + 2) * BK, K, p.rsB);
+    ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
+    gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
     compute(s & 1);
-    swrite((s + 1) & 1, (s + 1) * BK);
+    swrite((s + 1) & 1, kbeg + (s + 1) * BK);
     __syncthreads();
   }
   compute((nst - 1) & 1);
@@ -234,8 +234,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
       // form, lane/16 + 4r for the f64 one.
       const int64_t j = jb + (sizeof(T) == 4 ? 4 * g + r : g + 4 * r);
       if (iok && j < N) {
-        T* dst = Cr + 2 * int64_t(rc + p.csC[j]) + (odd ? 1 : 0);
-        *dst += val;
+        accumulate(Cr + 2 * int64_t(rc + p.csC[j]) + (odd ? 1 : 0), val, p.ksplit > 1);
       }
     }
   };
@@ -257,9 +256,9 @@ template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
           bool AF, bool BF>
 __global__ void __launch_bounds__(64 * WGM * WGN)
 k_contract_1m(KArgs<Idx> p) {
-  int tm, tn;
-  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
-  contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, tm, tn);
+  int tm, tn, kbeg, kend;
+  tile_and_krange(p, tm, tn, kbeg, kend);
+  contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, tm, tn, kbeg, kend);
 }
 
 // Persistent batched form: workgroup g takes items g, g + G, g + 2G, ... of
@@ -273,7 +272,7 @@ k_contract_1m_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __re
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const BatchItem w = items[it];
     const KArgs<Idx> p = inst[w.inst];
-    contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn);
+    contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn, 0, int(p.K));
     __syncthreads();   // the next tile rewrites LDS
   }
 }
@@ -332,7 +331,7 @@ hipError_t launch_simple(const GemmProblem& g, hipStream_t s) {
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN>
 hipError_t launch_1m(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
-  const dim3 grid(unsigned(a.tiles_m * a.tiles_n)), block(64 * WGM * WGN);
+  const dim3 grid(unsigned(a.tiles_m * a.tiles_n * a.ksplit)), block(64 * WGM * WGN);
   const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
   if (af && bf)
     hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, a);
@@ -427,9 +426,11 @@ size_t batch_args_size(bool idx32) {
   return idx32 ? sizeof(KArgs<int32_t>) : sizeof(KArgs<int64_t>);
 }
 
-void batch_make_args(const GemmProblem& g, int32_t kernel, void* out, int* tiles_m,
+void batch_make_args(const GemmProblem& g0, int32_t kernel, void* out, int* tiles_m,
                      int* tiles_n) {
   const KernelInfo& ki = kInfo[(kernel >= 0 && kernel < K_COUNT) ? kernel : 0];
+  GemmProblem g = g0;
+  g.split_k = 1;        // a batched launch covers whole tiles
   if (g.idx32) {
     const KArgs<int32_t> a = make_args<int32_t>(g, ki.bm, ki.bn);
     memcpy(out, &a, sizeof(a));

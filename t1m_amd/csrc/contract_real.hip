@@ -22,7 +22,7 @@ namespace {
 
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
           bool AF, bool BF>
-__device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, int tn) {
+__device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, int tn, int kbeg, int kend) {
   using Acc = typename Vec<T>::acc;
   constexpr int NT = 64 * WGM * WGN;
   constexpr int WTM = BM / WGM;          // rows per wave
@@ -45,7 +45,7 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
   const int wm = wid % WGM, wn = wid / WGM;
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
   const int64_t M = p.M, N = p.N;
-  const int K = int(p.K);
+  const int K = kend;     // contracted range [kbeg, kend) of this workgroup
   const T* __restrict__ A = static_cast<const T*>(p.A);
   const T* __restrict__ B = static_cast<const T*>(p.B);
 
@@ -134,22 +134,22 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
     }
   };
 
-  const int nst = (K + BK - 1) / BK;
-  ga.load_k(utid, 0, K, p.csA);
-  gb.load_k(utid, 0, K, p.rsB);
+  const int nst = (K - kbeg + BK - 1) / BK;
+  ga.load_k(utid, kbeg, K, p.csA);
+  gb.load_k(utid, kbeg, K, p.rsB);
   ga.load_data(A);
   gb.load_data(B);
-  ga.load_k(utid, BK, K, p.csA);
-  gb.load_k(utid, BK, K, p.rsB);
-  swrite(0, 0);
+  ga.load_k(utid, kbeg + BK, K, p.csA);
+  gb.load_k(utid, kbeg + BK, K, p.rsB);
+  swrite(0, kbeg);
   __syncthreads();
   for (int s = 0; s + 1 < nst; ++s) {
     ga.load_data(A);
     gb.load_data(B);
-    ga.load_k(utid, (s + 2) * BK, K, p.csA);
-    gb.load_k(utid, (s + 2) * BK, K, p.rsB);
+    ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
+    gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
     compute(s & 1);
-    swrite((s + 1) & 1, (s + 1) * BK);
+    swrite((s + 1) & 1, kbeg + (s + 1) * BK);
     __syncthreads();
   }
   compute((nst - 1) & 1);
@@ -163,7 +163,7 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
     for (int r = 0; r < 4; ++r) {
       // MFMA output row of (lane, r): 4(lane/16) + r (f32), lane/16 + 4r (f64).
       const int64_t j = jb + (sizeof(T) == 4 ? 4 * g + r : g + 4 * r);
-      if (iok && j < N) C[int64_t(rc + p.csC[j])] += alpha * v[r];
+      if (iok && j < N) accumulate(C + int64_t(rc + p.csC[j]), alpha * v[r], p.ksplit > 1);
     }
   };
 #pragma unroll
@@ -180,9 +180,9 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
 
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
 __global__ void __launch_bounds__(64 * WGM * WGN) k_contract_real(KArgs<Idx> p) {
-  int tm, tn;
-  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
-  contract_tile_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, tm, tn);
+  int tm, tn, kbeg, kend;
+  tile_and_krange(p, tm, tn, kbeg, kend);
+  contract_tile_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, tm, tn, kbeg, kend);
 }
 
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
@@ -192,7 +192,7 @@ k_contract_real_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const BatchItem w = items[it];
     const KArgs<Idx> p = inst[w.inst];
-    contract_tile_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn);
+    contract_tile_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn, 0, int(p.K));
     __syncthreads();
   }
 }
@@ -203,7 +203,7 @@ struct RealCfg {
   using KB = void (*)(const KArgs<Idx>*, const BatchItem*, int);
   template <bool AF, bool BF>
   static hipError_t launch_one(const KArgs<Idx>& a, hipStream_t s) {
-    const dim3 grid(unsigned(a.tiles_m * a.tiles_n)), block(64 * WGM * WGN);
+    const dim3 grid(unsigned(a.tiles_m * a.tiles_n * a.ksplit)), block(64 * WGM * WGN);
     hipLaunchKernelGGL((k_contract_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>), grid, block, 0, s, a);
     return hipGetLastError();
   }

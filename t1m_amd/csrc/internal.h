@@ -60,6 +60,8 @@ struct GemmProblem {
   int32_t b_kfast;   // stage B with lanes along k (else along n)
   int32_t idx32;     // offsets are int32 (else int64)
   int32_t dtype;     // t1m_dtype
+  int32_t split_k;   // 0: split K when the tiles cannot fill the CUs; 1: never
+  int32_t cus;       // compute units of the device (split-K sizing)
 };
 
 // Kernel families.

@@ -35,7 +35,18 @@ struct KArgs {
   int32_t conj_a, conj_b;
   int32_t a_mfast, b_kfast;
   int32_t tiles_m, tiles_n;
+  int32_t ksplit;    // workgroups per tile along K (1: no split)
+  int32_t kchunk;    // contracted extent per split, a multiple of the stage depth
 };
+
+// C[idx] += v; atomically when several workgroups share the tile (split K).
+template <typename T>
+__device__ inline void accumulate(T* dst, T v, bool atomic) {
+  if (atomic)
+    unsafeAtomicAdd(dst, v);
+  else
+    *dst += v;
+}
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
 // Renumber so that XCD x receives a contiguous range of logical tiles, and
@@ -49,9 +60,8 @@ constexpr int kGroupM = 4;
 #define T1M_XCD_REMAP 1
 #endif
 
-__device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
+__device__ inline void tile_coords_of(int w, int tiles_m, int tiles_n, int& tm, int& tn) {
   const int nwg = tiles_m * tiles_n;
-  int w = blockIdx.x;
   if (T1M_XCD_REMAP && nwg % kXcds == 0) w = (w % kXcds) * (nwg / kXcds) + w / kXcds;
   const int band = kGroupM * tiles_n;
   const int b0 = w / band * kGroupM;
@@ -139,8 +149,29 @@ struct TileGather {
   }
 };
 
+__device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
+  tile_coords_of(int(blockIdx.x), tiles_m, tiles_n, tm, tn);
+}
+
+// Tile and K range of the workgroup: the grid is tiles x ksplit, tile-major.
 template <typename Idx>
-KArgs<Idx> make_args(const GemmProblem& g, int bm, int bn) {
+__device__ inline void tile_and_krange(const KArgs<Idx>& p, int& tm, int& tn, int& kbeg,
+                                       int& kend) {
+  const int ntiles = p.tiles_m * p.tiles_n;
+  const int w = int(blockIdx.x) % ntiles;
+  const int split = int(blockIdx.x) / ntiles;
+  tile_coords_of(w, p.tiles_m, p.tiles_n, tm, tn);
+  kbeg = split * p.kchunk;
+  const int64_t e = int64_t(kbeg) + p.kchunk;
+  kend = e < p.K ? int(e) : int(p.K);
+}
+
+
+// Kernel arguments of problem g for bm x bn tiles with stage depth bk.  K is
+// split over several workgroups per tile when the tiles cannot fill the CUs
+// (fewer than half as many tiles as CUs) and each split keeps >= 8 stages.
+template <typename Idx>
+KArgs<Idx> make_args(const GemmProblem& g, int bm, int bn, int bk = 16) {
   KArgs<Idx> a;
   a.A = g.A; a.B = g.B; a.C = g.C;
   a.rsA = static_cast<const Idx*>(g.rsA);
@@ -155,6 +186,20 @@ KArgs<Idx> make_args(const GemmProblem& g, int bm, int bn) {
   a.a_mfast = g.a_mfast; a.b_kfast = g.b_kfast;
   a.tiles_m = bm > 0 ? int((g.M + bm - 1) / bm) : 0;
   a.tiles_n = bn > 0 ? int((g.N + bn - 1) / bn) : 0;
+  a.ksplit = 1;
+  a.kchunk = int(g.K);
+  const int64_t tiles = int64_t(a.tiles_m) * a.tiles_n;
+  const int cus = g.cus > 0 ? g.cus : 256;
+  if (g.split_k == 0 && bm > 0 && tiles > 0 && 2 * tiles < cus) {
+    const int64_t want = (2 * int64_t(cus) + tiles - 1) / tiles;     // ~2 waves of workgroups
+    const int64_t most = g.K / (8 * int64_t(bk));                     // >= 8 stages per split
+    const int64_t split = want < most ? want : most;
+    if (split > 1) {
+      const int64_t chunk = ((g.K + split - 1) / split + bk - 1) / bk * bk;
+      a.kchunk = int(chunk);
+      a.ksplit = int((g.K + chunk - 1) / chunk);
+    }
+  }
   return a;
 }
 

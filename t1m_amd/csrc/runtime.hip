@@ -685,6 +685,8 @@ int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
   g.a_mfast = plan->a_mfast; g.b_kfast = plan->b_kfast;
   g.idx32 = plan->idx32;
   g.dtype = P.dtype;
+  g.split_k = opt.deterministic ? 1 : 0;
+  g.cus = cu_count();
   if (opt.autotune && !plan->tuned.load() && (rc = autotune_plan(*plan, g, stream))) return rc;
   hipError_t e = launch_contract(g, plan->kernel, stream);
   if (e != hipSuccess)
@@ -715,6 +717,7 @@ void t1m_options_init(t1m_options* opt) {
   opt->synchronize = 1;
   opt->stream = nullptr;
   opt->autotune = 0;
+  opt->deterministic = 0;
 }
 
 const char* t1m_last_error(void) { return last_error(); }

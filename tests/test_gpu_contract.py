@@ -254,6 +254,39 @@ def test_real_contractions(dtype, idx64):
             assert err < (2e-6 if dtype == np.float32 else 1e-13), (la, lb, lc, kern, err)
 
 
+SPLIT_KERNELS = [(np.complex64, "c64_f32mfma_144x144", "native", "fp32"),
+                 (np.complex64, "c64_f32mfma_64x64", "native", "fp32"),
+                 (np.complex64, "c64_bf16x3_64x64", "bf16x3", "bf16x3"),
+                 (np.complex64, "c64_bf16x3_144x144", "bf16x3", "bf16x3"),
+                 (np.complex128, "c128_f64mfma_32x32", "native", "fp64"),
+                 (np.complex128, "c128_f64mfma_64x64", "native", "fp64"),
+                 (np.float32, "f32mfma_128x128", "native", "fp32"),
+                 (np.float64, "f64mfma_32x32", "native", "fp64")]
+
+
+@pytest.mark.parametrize("dtype,kern,prec,tolkey", SPLIT_KERNELS)
+def test_split_k_small_output_long_k(dtype, kern, prec, tolkey):
+    """cad,dcb->ab-like (benchmark.cpp:163-166): a C of one tile and K in the
+    thousands splits K over workgroups (atomic accumulation into C); the
+    deterministic option keeps one workgroup per tile and is bitwise
+    reproducible."""
+    rng = np.random.default_rng(30)
+    la, lb, lc = "cad", "dcb", "ab"
+    ext = {"a": 40, "b": 37, "c": 61, "d": 70}
+    A = rand(rng, [ext[x] for x in la], dtype)
+    B = rand(rng, [ext[x] for x in lb], dtype)
+    C0 = rand(rng, [ext[x] for x in lc], dtype)
+    want = oracle_einsum(A, la, B, lb, C0, lc, alpha=0.5 if np.dtype(dtype).kind == "f" else 0.5 + 1j)
+    alpha = 0.5 if np.dtype(dtype).kind == "f" else 0.5 + 1j
+    with forced(kern):
+        got = run(A, la, B, lb, C0, lc, precision=prec, alpha=alpha)
+        assert rel_err(got - C0, want - C0) < TOL[tolkey], "split"
+        d1 = run(A, la, B, lb, C0, lc, precision=prec, alpha=alpha, deterministic=True)
+        d2 = run(A, la, B, lb, C0, lc, precision=prec, alpha=alpha, deterministic=True)
+    assert rel_err(d1 - C0, want - C0) < TOL[tolkey], "deterministic"
+    np.testing.assert_array_equal(d1, d2)
+
+
 def test_conj_alpha_and_accumulate():
     rng = np.random.default_rng(8)
     la, lb, lc = "abcd", "dbef", "acef"

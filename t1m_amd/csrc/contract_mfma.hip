@@ -19,7 +19,10 @@
 //     first MFMA uses B's real parts with A_r's even real columns
 //     (s ? im : re), the second B's imaginary parts with A_r's odd real
 //     columns (s ? re : -im), s = row parity.  Two MFMAs therefore cover four
-//     complex k and the 1M expansion of A never leaves registers.
+//     complex k.  For FP32 the LDS tile of A holds both parities' operand
+//     pairs, (re, -im) and (im, re), written once per element, so the MFMAs
+//     read their operands straight from LDS; for FP64 the pair is selected
+//     in registers.
 //   * scatter epilogue: lane l holds re (even l) or im (odd l) of
 //     C[i][j]; C is read-modified-written through rsC/csC (C += alpha*A.B,
 //     t1m.h:543-548), so arbitrarily strided C works and rows with unit stride
@@ -36,10 +39,32 @@
 #include "kernel_common.h"
 
 #include <cstring>
+#include <type_traits>
 
 // K depth of one LDS stage for the 144 x 144 FP32 tiles.
 #ifndef T1M_BK144
 #define T1M_BK144 16
+#endif
+// LDS layout choices per precision (measured on c64 48^4 and c128 tiles):
+// pre-expanded A pays for FP32, whose MFMAs are short enough that the
+// per-chunk 1M selects and their hazard waits showed; FP64 MFMAs take twice
+// as long and the doubled A tile costs more than the selects.
+#ifndef T1M_AEXP32
+#define T1M_AEXP32 1
+#endif
+#ifndef T1M_AEXP64
+#define T1M_AEXP64 0
+#endif
+#ifndef T1M_BSWZ32
+#define T1M_BSWZ32 1
+#endif
+#ifndef T1M_BSWZ64
+#define T1M_BSWZ64 1
+#endif
+// Mid-stage LDS writes (next stage written after half of this stage's
+// MFMAs) for tiles below 144 x 144, which have the registers to spare.
+#ifndef T1M_MIDW
+#define T1M_MIDW 1
 #endif
 
 namespace t1m_hip {
@@ -60,15 +85,36 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   constexpr bool XT = NBT % WGN != 0;    // one shared column block
   constexpr int XB = XT ? (MB + WGN - 1) / WGN : 0;
   constexpr int XB_ALLOC = XT ? XB : 1;
-  // Row pitch padding (complex elements) so the four lane groups of a
-  // fragment read hit different LDS banks.
-  constexpr int PAD = sizeof(T) == 4 ? 8 : 4;
-  constexpr int LDA = BM + PAD, LDB = BN + PAD;
+  constexpr bool AEXP = sizeof(T) == 4 ? T1M_AEXP32 != 0 : T1M_AEXP64 != 0;
+  constexpr bool BSWZ = sizeof(T) == 4 ? T1M_BSWZ32 != 0 : T1M_BSWZ64 != 0;
+  constexpr bool MIDW = T1M_MIDW != 0 && BM * BN < 144 * 144;
+  // LDS layouts.  A holds each complex element twice, pre-expanded for the
+  // two row parities of 1M: (re, -im) for even lanes, (im, re) for odd ones,
+  // so the MFMA operands come straight from the LDS reads.  Both tiles are
+  // XOR-swizzled by k: element (k, x) sits in slot x ^ sw(k), sw(k) =
+  // (k/2) mod (half a bank sweep), and the row pitch is an odd multiple of
+  // 128 bytes.  A 16-lane read group (8 A units or 16 B units, aligned) stays
+  // one contiguous bank sweep, and a gather along k writes 256 consecutive
+  // bytes of banks per 256 bytes stored.
+  struct alignas(2 * sizeof(C2)) AExp {
+    C2 e, o;
+  };
+  using AU = std::conditional_t<AEXP, AExp, C2>;
+  constexpr int SA = 256 / int(sizeof(AU));   // units per 256-byte bank sweep
+  constexpr int SB = 256 / int(sizeof(C2));
+  constexpr int PADA = AEXP ? ((SA / 2 - BM % SA) % SA + SA) % SA : (sizeof(T) == 4 ? 8 : 4);
+  constexpr int PADB = BSWZ ? (((SB / 2 - BN % SB) + SB) % SB) : (sizeof(T) == 4 ? 8 : 4);
+  constexpr int LDA = BM + PADA, LDB = BN + PADB;
+  static_assert(!AEXP || (LDA * int(sizeof(AU))) % 256 == 128, "A pitch");
+  static_assert(!BSWZ || (LDB * int(sizeof(C2))) % 256 == 128, "B pitch");
   static_assert(WTM % 8 == 0 && BN % 16 == 0, "tile shape");
   static_assert(NBT - NBW * WGN <= 1, "at most one shared column block");
   static_assert(BK % 4 == 0, "k chunk");
+  // A read group spans 8 aligned units, a B group 16.
+  auto swa = [](int k) { return AEXP ? (k >> 1) & (SA / 2 - 1 < 7 ? SA / 2 - 1 : 7) : 0; };
+  auto swb = [](int k) { return BSWZ ? (k >> 1) & (SB / 2 - 1 < 15 ? SB / 2 - 1 : 15) : 0; };
 
-  __shared__ C2 sA[2][BK][LDA];
+  __shared__ AU sA[2][BK][LDA];
   __shared__ C2 sB[2][BK][LDB];
 
   const int tid = threadIdx.x;
@@ -98,12 +144,24 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
 #pragma unroll
     for (int u = 0; u < GA::E; ++u) {
       const unsigned e = GA::elem(utid, u);
-      if (GA::valid(utid, u)) sA[buf][GA::kc(e)][GA::xc(e)] = ga.val[u];
+      const int k = GA::kc(e);
+      if (GA::valid(utid, u)) {
+        const C2 v = ga.val[u];
+        if constexpr (AEXP) {
+          AExp w;
+          w.e.x = v.x; w.e.y = -v.y;
+          w.o.x = v.y; w.o.y = v.x;
+          sA[buf][k][GA::xc(e) ^ swa(k)] = w;
+        } else {
+          sA[buf][k][GA::xc(e)] = v;
+        }
+      }
     }
 #pragma unroll
     for (int u = 0; u < GB::E; ++u) {
       const unsigned e = GB::elem(utid, u);
-      if (GB::valid(utid, u)) sB[buf][GB::kc(e)][GB::xc(e)] = gb.val[u];
+      const int k = GB::kc(e);
+      if (GB::valid(utid, u)) sB[buf][k][GB::xc(e) ^ swb(k)] = gb.val[u];
     }
   };
 
@@ -129,32 +187,52 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
     xmb[x] = mb < MB ? mb : MB - 1;
   }
 
-  // Fragments of one 4-k chunk, raw (re, im) pairs as read from LDS.
+  // Fragments of one 4-k chunk: A as the two 1M operands of this lane's
+  // row parity (AEXP) or as the raw (re, im) pair, B as (re, im).
   struct Frag {
     C2 a[MB];
     C2 b[NBW];
     C2 ax[XB_ALLOC];
     C2 bx;
   };
-  auto load_frag = [&](int buf, int kc, Frag& f) {
-    const int q = kc * 4 + g;
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) f.a[mb] = sA[buf][q][am0 + mb * 8];
-#pragma unroll
-    for (int nb = 0; nb < NBW; ++nb) f.b[nb] = sB[buf][q][bn0 + nb * 16];
-    if constexpr (XT) {
-      f.bx = sB[buf][q][bnx];
-#pragma unroll
-      for (int x = 0; x < XB; ++x) f.ax[x] = sA[buf][q][am0 + xmb[x] * 8];
+  auto aread = [&](int buf, int q, int row0) -> C2 {
+    const int x = row0 + (((l16 >> 1) ^ swa(q)) - (l16 >> 1));
+    if constexpr (AEXP) {
+      const C2* u = reinterpret_cast<const C2*>(&sA[buf][q][x]);
+      return u[odd ? 1 : 0];
+    } else {
+      return sA[buf][q][x];
     }
   };
-  // 1M expansion in registers, then two MFMAs per tile: B's real parts
-  // against A_r's even real columns, B's imaginary parts against the odd.
+  auto load_frag = [&](int buf, int kc, Frag& f) {
+    const int q = kc * 4 + g;
+    const int bsw = (l16 ^ swb(q)) - l16;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) f.a[mb] = aread(buf, q, am0 + mb * 8);
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb) f.b[nb] = sB[buf][q][bn0 + bsw + nb * 16];
+    if constexpr (XT) {
+      f.bx = sB[buf][q][bnx + bsw];
+#pragma unroll
+      for (int x = 0; x < XB; ++x) f.ax[x] = aread(buf, q, am0 + xmb[x] * 8);
+    }
+  };
+  // Two MFMAs per tile: B's real parts against A_r's even real columns,
+  // B's imaginary parts against the odd.
+  auto operands = [&](const C2& a, T& re, T& im) {
+    if constexpr (AEXP) {
+      re = a.x;
+      im = a.y;
+    } else {
+      re = odd ? a.y : a.x;    // A_r[2i+s][2q]
+      im = odd ? a.x : -a.y;   // A_r[2i+s][2q+1]
+    }
+  };
   auto mma = [&](const Frag& f) {
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
-      const T re = odd ? f.a[mb].y : f.a[mb].x;    // A_r[2i+s][2q]
-      const T im = odd ? f.a[mb].x : -f.a[mb].y;   // A_r[2i+s][2q+1]
+      T re, im;
+      operands(f.a[mb], re, im);
 #pragma unroll
       for (int nb = 0; nb < NBW; ++nb) {
         acc[mb][nb] = mfma16(f.b[nb].x, re, acc[mb][nb]);
@@ -164,8 +242,8 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
     if constexpr (XT) {
 #pragma unroll
       for (int x = 0; x < XB; ++x) {
-        const T re = odd ? f.ax[x].y : f.ax[x].x;
-        const T im = odd ? f.ax[x].x : -f.ax[x].y;
+        T re, im;
+        operands(f.ax[x], re, im);
         accx[x] = mfma16(f.bx.x, re, accx[x]);
         accx[x] = mfma16(f.bx.y, im, accx[x]);
       }
@@ -174,7 +252,10 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   // Chunk kc + 1's LDS reads are issued before chunk kc's MFMAs (the
   // scheduling barriers keep the compiler from sinking them), so a wave
   // never waits on LDS latency inside a stage.
-  auto compute = [&](int buf) {
+  // `mid` runs after the first half of the stage's MFMAs are issued: the
+  // main loop puts the next stage's LDS writes there, so they overlap the
+  // matrix pipe instead of idling it before the barrier.
+  auto compute = [&](int buf, auto&& mid) {
     constexpr int KCH = BK / 4;
     Frag f[2];
     load_frag(buf, 0, f[0]);
@@ -184,6 +265,10 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
       __builtin_amdgcn_sched_barrier(0);
       mma(f[kc & 1]);
       __builtin_amdgcn_sched_barrier(0);
+      if (kc == (MIDW ? KCH / 2 - 1 : KCH - 1)) {
+        mid();
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   };
 
@@ -208,11 +293,10 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
     gb.load_data(B);
     ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
     gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
-    compute(s & 1);
-    swrite((s + 1) & 1, kbeg + (s + 1) * BK);
+    compute(s & 1, [&] { swrite((s + 1) & 1, kbeg + (s + 1) * BK); });
     __syncthreads();
   }
-  compute((nst - 1) & 1);
+  compute((nst - 1) & 1, [] {});
 
   // ---- epilogue: C[rsC[i] + csC[j]] += alpha * acc -----------------------
   T* __restrict__ Cr = static_cast<T*>(p.C);

@@ -21,9 +21,37 @@
 //
 // LDS holds one 32-bit word per complex element and plane (hi, lo), rows of
 // m (for A) / n (for B) with the k index contiguous, so a lane's fragment is
-// one 16-byte read per plane.
+// one 16-byte read per plane.  On the 144 x 144 tiles A's rows are stored
+// already expanded (real rows 2i + s), so the fragment reads need no
+// per-reader perm / xor.  Operands gathered along m / n are staged in quads
+// of four consecutive k (one 16-byte LDS store per plane and row).
 #include "internal.h"
 #include "kernel_common.h"
+
+#include <type_traits>
+
+// Layout switches, measured on complex<float> 48^4 contractions in the four
+// gather-direction combinations: expanded A (+11 % on 144 x 144 tiles, a
+// loss on 64 x 64 where the larger LDS tile costs occupancy), quad staging
+// of A (+5..15 %) and of B (64 x 64 tiles only), writes after the stage's
+// MFMAs (0) rather than interleaved with them (1: at half, 2: spread), and
+// the chunk-swizzled unpadded rows (1) that helped one direction combination
+// and lost on two others.
+#ifndef T1M_BFAEXP
+#define T1M_BFAEXP 1
+#endif
+#ifndef T1M_BFQA
+#define T1M_BFQA 1
+#endif
+#ifndef T1M_BFQB
+#define T1M_BFQB 1
+#endif
+#ifndef T1M_BFMID
+#define T1M_BFMID 0
+#endif
+#ifndef T1M_BFSWZ
+#define T1M_BFSWZ 0
+#endif
 
 namespace t1m_hip {
 
@@ -51,6 +79,68 @@ __device__ inline f32x4 mfma_bf16(const u32x4& a, const u32x4& b, f32x4 c) {
       __builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
+// Gather of a BX x BK tile along x (the operand's unit-stride direction) in
+// quads: lane work unit (x, kq) holds the four consecutive k = 4kq..4kq+3 of
+// row x, so it can store them as one 16-byte LDS write per plane into the
+// k-contiguous rows the MFMA fragments read.  Lanes run along x, so each of
+// the four global loads of a unit is coalesced across lanes.
+template <typename Idx, int BX, int BK, int NT>
+struct QuadGather {
+  static constexpr int NQ = BX * (BK / 4);
+  static constexpr int E = (NQ + NT - 1) / NT;
+  static constexpr bool RAGGED = (NQ % NT) != 0;
+  Idx xoff[E];
+  Idx koff[E][4];
+  float2 val[E][4];
+  uint32_t xok;
+
+  __device__ static constexpr int xc(unsigned e) { return int(e % unsigned(BX)); }
+  __device__ static constexpr int kq(unsigned e) { return int(e / unsigned(BX)); }
+  __device__ static bool valid(unsigned tid, int u) {
+    return !RAGGED || u + 1 < E || tid + unsigned(NT) * u < unsigned(NQ);
+  }
+  __device__ static unsigned elem(unsigned tid, int u) {
+    return valid(tid, u) ? tid + unsigned(NT) * u : 0u;
+  }
+  __device__ void init(unsigned tid, int64_t x0, int64_t X, const Idx* xmap) {
+    xok = 0;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int64_t xg = x0 + xc(elem(tid, u));
+      xoff[u] = xmap[xg < X ? xg : X - 1];
+      xok |= ((xg < X && valid(tid, u)) ? 1u : 0u) << u;
+    }
+  }
+  __device__ void load_k(unsigned tid, int p0, int K, const Idx* kmap) {
+#pragma unroll
+    for (int u = 0; u < E; ++u)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int kg = p0 + 4 * kq(elem(tid, u)) + t;
+        koff[u][t] = kmap[kg < K ? kg : K - 1];
+      }
+  }
+  __device__ void load_data(const float2* __restrict__ base) {
+#pragma unroll
+    for (int u = 0; u < E; ++u)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) val[u][t] = base[xoff[u] + koff[u][t]];
+  }
+  __device__ void finish_one(unsigned tid, int p0, int K, float sign, int u) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int kg = p0 + 4 * kq(elem(tid, u)) + t;
+      const bool ok = ((xok >> u) & 1u) && kg < K;
+      val[u][t].x = ok ? val[u][t].x : 0.f;
+      val[u][t].y = ok ? sign * val[u][t].y : 0.f;
+    }
+  }
+  __device__ void finish(unsigned tid, int p0, int K, float sign) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) finish_one(tid, p0, K, sign, u);
+  }
+};
+
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
 __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm, int tn, int kbeg, int kend) {
   constexpr int NT = 64 * WGM * WGN;
@@ -61,13 +151,25 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   constexpr bool XT = NBT % WGN != 0;
   constexpr int XB = XT ? (MB + WGN - 1) / WGN : 0;
   constexpr int XB_ALLOC = XT ? XB : 1;
-  constexpr int PITCH = BK + 4;          // words per LDS row
+  // LDS rows of BK words.  SWZ: no padding, the 16-byte chunks of row r are
+  // permuted, chunk c stored at c ^ ((r / 2) mod 4); the fragment reads
+  // (16 consecutive rows, one chunk each) and the stores of both gather
+  // directions then spread over the banks.  Otherwise rows are padded by 4
+  // words.
+  constexpr bool SWZ = T1M_BFSWZ != 0;
+  constexpr int PITCH = SWZ ? BK : BK + 4;   // words per LDS row
+  static_assert(!SWZ || BK == 16, "swizzle assumes four chunks per row");
+  auto col = [](int r, int k) { return SWZ ? ((((k >> 2) ^ ((r >> 1) & 3)) << 2) | (k & 3)) : k; };
   static_assert(BK % 16 == 0, "16 complex k per MFMA");
   static_assert(WTM % 8 == 0 && BN % 16 == 0, "tile shape");
   static_assert(NBT - NBW * WGN <= 1, "at most one shared column block");
 
-  __shared__ __attribute__((aligned(16))) uint32_t sAh[2][BM][PITCH];
-  __shared__ __attribute__((aligned(16))) uint32_t sAl[2][BM][PITCH];
+  // AEXP: A's planes hold the 1M-expanded real rows 2i + s, written once per
+  // element, instead of the raw (re, im) words expanded by every reader.
+  constexpr bool AEXP = T1M_BFAEXP != 0 && BM >= 144;
+  constexpr int AROWS = AEXP ? 2 * BM : BM;
+  __shared__ __attribute__((aligned(16))) uint32_t sAh[2][AROWS][PITCH];
+  __shared__ __attribute__((aligned(16))) uint32_t sAl[2][AROWS][PITCH];
   __shared__ __attribute__((aligned(16))) uint32_t sBh[2][BN][PITCH];
   __shared__ __attribute__((aligned(16))) uint32_t sBl[2][BN][PITCH];
 
@@ -81,8 +183,15 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   const float2* __restrict__ B = static_cast<const float2*>(p.B);
 
   const unsigned utid = unsigned(tid);
-  using GA = TileGather<float2, Idx, BM, BK, NT, AF>;
-  using GB = TileGather<float2, Idx, BN, BK, NT, BF>;
+  // Gathers along x store quads (16-byte writes); gathers along k store
+  // single words, consecutive lanes filling consecutive words of a row.
+  // (Measured: quads pay for A at both tile sizes and for B only on the
+  // 64 x 64 tiles.)
+  constexpr bool QA = AF && T1M_BFQA != 0, QB = BF && T1M_BFQB != 0 && BM < 144;
+  using GA = std::conditional_t<QA, QuadGather<Idx, BM, BK, NT>,
+                                TileGather<float2, Idx, BM, BK, NT, AF>>;
+  using GB = std::conditional_t<QB, QuadGather<Idx, BN, BK, NT>,
+                                TileGather<float2, Idx, BN, BK, NT, BF>>;
   GA ga;
   GB gb;
   ga.init(utid, i0, M, p.rsA);
@@ -90,29 +199,93 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   const float sa = p.conj_a ? -1.f : 1.f;
   const float sb = p.conj_b ? -1.f : 1.f;
 
-  auto swrite = [&](int buf, int p0) {
-    ga.finish(utid, p0, K, sa);
-    gb.finish(utid, p0, K, sb);
+  // Expanded A rows: 2x holds (re, -im), 2x + 1 holds (im, re).
+  auto exp0 = [](uint32_t w) { return w ^ 0x80000000u; };
+  auto exp1 = [](uint32_t w) { return __builtin_amdgcn_alignbit(w, w, 16); };
+  // Stage one gathered unit (an element, or a quad of four k) of A / B into
+  // LDS buffer buf: zero / conjugate, split into hi and lo words, store.
+  auto write_a = [&](int buf, int p0, int u) {
+    ga.finish_one(utid, p0, K, sa, u);
+    const unsigned e = GA::elem(utid, u);
+    if constexpr (QA) {
+      u32x4 hi, lo;
 #pragma unroll
-    for (int u = 0; u < GA::E; ++u) {
-      const unsigned e = GA::elem(utid, u);
+      for (int t = 0; t < 4; ++t) {
+        uint32_t h, l;
+        split_pair(ga.val[u][t].x, ga.val[u][t].y, h, l);
+        hi[t] = h;
+        lo[t] = l;
+      }
+      if (GA::valid(utid, u)) {
+        const int x = GA::xc(e), kw = 4 * GA::kq(e);
+        if constexpr (AEXP) {
+          u32x4 h0, h1, l0, l1;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            h0[t] = exp0(hi[t]); h1[t] = exp1(hi[t]);
+            l0[t] = exp0(lo[t]); l1[t] = exp1(lo[t]);
+          }
+          *reinterpret_cast<u32x4*>(&sAh[buf][2 * x][col(2 * x, kw)]) = h0;
+          *reinterpret_cast<u32x4*>(&sAh[buf][2 * x + 1][col(2 * x + 1, kw)]) = h1;
+          *reinterpret_cast<u32x4*>(&sAl[buf][2 * x][col(2 * x, kw)]) = l0;
+          *reinterpret_cast<u32x4*>(&sAl[buf][2 * x + 1][col(2 * x + 1, kw)]) = l1;
+        } else {
+          *reinterpret_cast<u32x4*>(&sAh[buf][x][col(x, kw)]) = hi;
+          *reinterpret_cast<u32x4*>(&sAl[buf][x][col(x, kw)]) = lo;
+        }
+      }
+    } else {
       uint32_t hi, lo;
       split_pair(ga.val[u].x, ga.val[u].y, hi, lo);
       if (GA::valid(utid, u)) {
-        sAh[buf][GA::xc(e)][GA::kc(e)] = hi;
-        sAl[buf][GA::xc(e)][GA::kc(e)] = lo;
+        const int x = GA::xc(e), k = GA::kc(e);
+        if constexpr (AEXP) {
+          sAh[buf][2 * x][col(2 * x, k)] = exp0(hi);
+          sAh[buf][2 * x + 1][col(2 * x + 1, k)] = exp1(hi);
+          sAl[buf][2 * x][col(2 * x, k)] = exp0(lo);
+          sAl[buf][2 * x + 1][col(2 * x + 1, k)] = exp1(lo);
+        } else {
+          sAh[buf][x][col(x, k)] = hi;
+          sAl[buf][x][col(x, k)] = lo;
+        }
       }
     }
+  };
+  auto write_b = [&](int buf, int p0, int u) {
+    gb.finish_one(utid, p0, K, sb, u);
+    const unsigned e = GB::elem(utid, u);
+    if constexpr (QB) {
+      u32x4 hi, lo;
 #pragma unroll
-    for (int u = 0; u < GB::E; ++u) {
-      const unsigned e = GB::elem(utid, u);
+      for (int t = 0; t < 4; ++t) {
+        uint32_t h, l;
+        split_pair(gb.val[u][t].x, gb.val[u][t].y, h, l);
+        hi[t] = h;
+        lo[t] = l;
+      }
+      if (GB::valid(utid, u)) {
+        const int x = GB::xc(e), kw = 4 * GB::kq(e);
+        *reinterpret_cast<u32x4*>(&sBh[buf][x][col(x, kw)]) = hi;
+        *reinterpret_cast<u32x4*>(&sBl[buf][x][col(x, kw)]) = lo;
+      }
+    } else {
       uint32_t hi, lo;
       split_pair(gb.val[u].x, gb.val[u].y, hi, lo);
       if (GB::valid(utid, u)) {
-        sBh[buf][GB::xc(e)][GB::kc(e)] = hi;
-        sBl[buf][GB::xc(e)][GB::kc(e)] = lo;
+        const int x = GB::xc(e), k = GB::kc(e);
+        sBh[buf][x][col(x, k)] = hi;
+        sBl[buf][x][col(x, k)] = lo;
       }
     }
+  };
+  constexpr int NU = GA::E + GB::E;
+  auto write_unit = [&](int buf, int p0, int i) {
+    if (i < GA::E) write_a(buf, p0, i);
+    else write_b(buf, p0, i - GA::E);
+  };
+  auto swrite = [&](int buf, int p0) {
+#pragma unroll
+    for (int i = 0; i < NU; ++i) write_unit(buf, p0, i);
   };
 
   f32x4 acc[MB][NBW];
@@ -146,49 +319,70 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
     for (int t = 0; t < 4; ++t) r[t] = __builtin_amdgcn_perm(w[t], w[t], psel) ^ pxor;
     return r;
   };
-  auto ldsA = [&](const uint32_t (*plane)[PITCH], int row, int kw) {
-    return expand(*reinterpret_cast<const u32x4*>(&plane[row][kw]));
+  // Word column of this lane's chunk g in the rows it reads: A rows are
+  // 2(i0 + 8mb) + l16 (expanded) or i0 + 8mb + l16/2, B rows j0 + 16nb +
+  // l16, with i0, j0 multiples of 8, so the swizzle is the same for every
+  // row block.
+  const int colA = col(AEXP ? l16 : l16 >> 1, 4 * g);
+  const int colB = col(l16, 4 * g);
+  auto ldsA = [&](const uint32_t (*plane)[PITCH], int row) {
+    if constexpr (AEXP)
+      return *reinterpret_cast<const u32x4*>(&plane[2 * row + (odd ? 1 : 0)][colA]);
+    else
+      return expand(*reinterpret_cast<const u32x4*>(&plane[row][colA]));
   };
-  auto ldsB = [&](const uint32_t (*plane)[PITCH], int row, int kw) {
-    return *reinterpret_cast<const u32x4*>(&plane[row][kw]);
+  auto ldsB = [&](const uint32_t (*plane)[PITCH], int row) {
+    return *reinterpret_cast<const u32x4*>(&plane[row][colB]);
   };
 
   // B fragments stay in registers for a k-chunk; A fragments stream per row
   // block.  Three products per tile: lo*hi and hi*lo first, then hi*hi.
-  auto compute = [&](int buf) {
+  // The next stage's LDS writes (NU units) go after the stage's MFMAs
+  // (T1M_BFMID 0), or inside the MFMA stream: all after row-block group
+  // G/2 - 1 (1), or unit i after group S + i (G - S) / NU, S = G / 2 (2).
+  constexpr int G = MB + (XT ? XB : 0);
+  constexpr int S = T1M_BFMID == 0 ? G : G / 2;
+  auto slot = [&](int i) { return T1M_BFMID == 1 ? S - 1 : S + (i * (G - S)) / NU; };
+  auto compute = [&](int buf, int wbuf, int p0, bool wr) {
+    static_assert(BK == 16, "one MFMA k-chunk per stage");
+    auto after_group = [&](int m) {
+      if (!wr) return;
 #pragma unroll
-    for (int kc = 0; kc < BK / 16; ++kc) {
-      const int kw = kc * 16 + 4 * g;
-      u32x4 bh[NBW], bl[NBW];
+      for (int i = 0; i < NU; ++i)
+        if (slot(i) == m) write_unit(wbuf, p0, i);
+    };
+    u32x4 bh[NBW], bl[NBW];
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb) {
+      bh[nb] = ldsB(sBh[buf], bn0 + nb * 16);
+      bl[nb] = ldsB(sBl[buf], bn0 + nb * 16);
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      const u32x4 ah = ldsA(sAh[buf], am0 + mb * 8);
+      const u32x4 al = ldsA(sAl[buf], am0 + mb * 8);
 #pragma unroll
       for (int nb = 0; nb < NBW; ++nb) {
-        bh[nb] = ldsB(sBh[buf], bn0 + nb * 16, kw);
-        bl[nb] = ldsB(sBl[buf], bn0 + nb * 16, kw);
+        acc[mb][nb] = mfma_bf16(bl[nb], ah, acc[mb][nb]);
+        acc[mb][nb] = mfma_bf16(bh[nb], al, acc[mb][nb]);
+        acc[mb][nb] = mfma_bf16(bh[nb], ah, acc[mb][nb]);
       }
+      after_group(mb);
+    }
+    if constexpr (XT) {
+      const u32x4 xbh = ldsB(sBh[buf], bnx);
+      const u32x4 xbl = ldsB(sBl[buf], bnx);
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-        const u32x4 ah = ldsA(sAh[buf], am0 + mb * 8, kw);
-        const u32x4 al = ldsA(sAl[buf], am0 + mb * 8, kw);
-#pragma unroll
-        for (int nb = 0; nb < NBW; ++nb) {
-          acc[mb][nb] = mfma_bf16(bl[nb], ah, acc[mb][nb]);
-          acc[mb][nb] = mfma_bf16(bh[nb], al, acc[mb][nb]);
-          acc[mb][nb] = mfma_bf16(bh[nb], ah, acc[mb][nb]);
-        }
-      }
-      if constexpr (XT) {
-        const u32x4 xbh = ldsB(sBh[buf], bnx, kw);
-        const u32x4 xbl = ldsB(sBl[buf], bnx, kw);
-#pragma unroll
-        for (int x = 0; x < XB; ++x) {
-          const u32x4 ah = ldsA(sAh[buf], am0 + xmb[x] * 8, kw);
-          const u32x4 al = ldsA(sAl[buf], am0 + xmb[x] * 8, kw);
-          accx[x] = mfma_bf16(xbl, ah, accx[x]);
-          accx[x] = mfma_bf16(xbh, al, accx[x]);
-          accx[x] = mfma_bf16(xbh, ah, accx[x]);
-        }
+      for (int x = 0; x < XB; ++x) {
+        const u32x4 ah = ldsA(sAh[buf], am0 + xmb[x] * 8);
+        const u32x4 al = ldsA(sAl[buf], am0 + xmb[x] * 8);
+        accx[x] = mfma_bf16(xbl, ah, accx[x]);
+        accx[x] = mfma_bf16(xbh, al, accx[x]);
+        accx[x] = mfma_bf16(xbh, ah, accx[x]);
+        after_group(MB + x);
       }
     }
+    if (wr && T1M_BFMID == 0) swrite(wbuf, p0);
   };
 
   const int nst = (K - kbeg + BK - 1) / BK;
@@ -209,11 +403,10 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
     gb.load_data(B);
     ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
     gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
-    compute(s & 1);
-    swrite((s + 1) & 1, kbeg + (s + 1) * BK);
+    compute(s & 1, (s + 1) & 1, kbeg + (s + 1) * BK, true);
     __syncthreads();
   }
-  compute((nst - 1) & 1);
+  compute((nst - 1) & 1, 0, 0, false);
 
   float* __restrict__ Cr = static_cast<float*>(p.C);
   const bool cplx_alpha = p.alpha_im != 0.0;

@@ -138,14 +138,16 @@ struct TileGather {
   }
   // Zero out-of-range entries and conjugate if asked (sign = -1).
   template <typename T>
+  __device__ void finish_one(unsigned tid, int p0, int K, T sign, int u) {
+    const int kg = p0 + kc(elem(tid, u));
+    const bool ok = ((xok >> u) & 1u) && kg < K;
+    val[u].x = ok ? val[u].x : T(0);
+    val[u].y = ok ? sign * val[u].y : T(0);
+  }
+  template <typename T>
   __device__ void finish(unsigned tid, int p0, int K, T sign) {
 #pragma unroll
-    for (int u = 0; u < E; ++u) {
-      const int kg = p0 + kc(elem(tid, u));
-      const bool ok = ((xok >> u) & 1u) && kg < K;
-      val[u].x = ok ? val[u].x : T(0);
-      val[u].y = ok ? sign * val[u].y : T(0);
-    }
+    for (int u = 0; u < E; ++u) finish_one(tid, p0, K, sign, u);
   }
 };
 

@@ -66,6 +66,16 @@
 #ifndef T1M_MIDW
 #define T1M_MIDW 1
 #endif
+// Fine-grained operand pipelining (A operands T1M_FINE_LA row groups ahead).
+#ifndef T1M_FINE
+#define T1M_FINE 1
+#endif
+#ifndef T1M_FINE_LA
+#define T1M_FINE_LA 2
+#endif
+#ifndef T1M_FINE_MIDW
+#define T1M_FINE_MIDW 1
+#endif
 
 namespace t1m_hip {
 
@@ -87,7 +97,8 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   constexpr int XB_ALLOC = XT ? XB : 1;
   constexpr bool AEXP = sizeof(T) == 4 ? T1M_AEXP32 != 0 : T1M_AEXP64 != 0;
   constexpr bool BSWZ = sizeof(T) == 4 ? T1M_BSWZ32 != 0 : T1M_BSWZ64 != 0;
-  constexpr bool MIDW = T1M_MIDW != 0 && BM * BN < 144 * 144;
+  constexpr bool FINE = T1M_FINE != 0;
+  constexpr bool MIDW = FINE ? T1M_FINE_MIDW != 0 : (T1M_MIDW != 0 && BM * BN < 144 * 144);
   // LDS layouts.  A holds each complex element twice, pre-expanded for the
   // two row parities of 1M: (re, -im) for even lanes, (im, re) for odd ones,
   // so the MFMA operands come straight from the LDS reads.  Both tiles are
@@ -255,19 +266,82 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   // `mid` runs after the first half of the stage's MFMAs are issued: the
   // main loop puts the next stage's LDS writes there, so they overlap the
   // matrix pipe instead of idling it before the barrier.
-  auto compute = [&](int buf, auto&& mid) {
-    constexpr int KCH = BK / 4;
-    Frag f[2];
-    load_frag(buf, 0, f[0]);
+  // Fine-grained form: B fragments one chunk ahead, A operands LA row
+  // groups ahead of the MFMAs that use them (a rolling window over the
+  // KCH x NG (chunk, row group) steps of the stage), which needs far fewer
+  // registers than a whole chunk of fragments in flight.
+  constexpr int KCH = BK / 4;
+  constexpr int NG = MB + (XT ? XB : 0);     // row groups per chunk
+  constexpr int TOT = KCH * NG;
+  constexpr int LA = T1M_FINE_LA;
+  struct BFr {
+    C2 b[NBW];
+    C2 bx;
+  };
+  auto load_b = [&](int buf, int kc, BFr& f) {
+    const int q = kc * 4 + g;
+    const int bsw = (l16 ^ swb(q)) - l16;
 #pragma unroll
-    for (int kc = 0; kc < KCH; ++kc) {
-      if (kc + 1 < KCH) load_frag(buf, kc + 1, f[(kc + 1) & 1]);
+    for (int nb = 0; nb < NBW; ++nb) f.b[nb] = sB[buf][q][bn0 + bsw + nb * 16];
+    if constexpr (XT) f.bx = sB[buf][q][bnx + bsw];
+  };
+  auto load_a = [&](int buf, int t) -> C2 {
+    const int kc = t / NG, grp = t % NG;
+    const int q = kc * 4 + g;
+    if (grp < MB) return aread(buf, q, am0 + grp * 8);
+    return aread(buf, q, am0 + xmb[XT ? grp - MB : 0] * 8);
+  };
+  auto mma_grp = [&](int grp, const C2& a, const BFr& f) {
+    T re, im;
+    operands(a, re, im);
+    if (grp < MB) {
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) {
+        acc[grp][nb] = mfma16(f.b[nb].x, re, acc[grp][nb]);
+        acc[grp][nb] = mfma16(f.b[nb].y, im, acc[grp][nb]);
+      }
+    } else if constexpr (XT) {
+      accx[grp - MB] = mfma16(f.bx.x, re, accx[grp - MB]);
+      accx[grp - MB] = mfma16(f.bx.y, im, accx[grp - MB]);
+    }
+  };
+  auto compute_fine = [&](int buf, auto&& mid) {
+    BFr fb[2];
+    C2 aw[LA + 1];
+    load_b(buf, 0, fb[0]);
+#pragma unroll
+    for (int t = 0; t < LA; ++t) aw[t] = load_a(buf, t);
+#pragma unroll
+    for (int t = 0; t < TOT; ++t) {
+      const int kc = t / NG, grp = t % NG;
+      if (grp == 0 && kc + 1 < KCH) load_b(buf, kc + 1, fb[(kc + 1) & 1]);
+      if (t + LA < TOT) aw[(t + LA) % (LA + 1)] = load_a(buf, t + LA);
       __builtin_amdgcn_sched_barrier(0);
-      mma(f[kc & 1]);
+      mma_grp(grp, aw[t % (LA + 1)], fb[kc & 1]);
       __builtin_amdgcn_sched_barrier(0);
-      if (kc == (MIDW ? KCH / 2 - 1 : KCH - 1)) {
+      if (MIDW && t == TOT / 2 - 1) {
         mid();
         __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    if (!MIDW) mid();
+  };
+  auto compute = [&](int buf, auto&& mid) {
+    if constexpr (FINE) {
+      compute_fine(buf, mid);
+    } else {
+      Frag f[2];
+      load_frag(buf, 0, f[0]);
+#pragma unroll
+      for (int kc = 0; kc < KCH; ++kc) {
+        if (kc + 1 < KCH) load_frag(buf, kc + 1, f[(kc + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(f[kc & 1]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (kc == (MIDW ? KCH / 2 - 1 : KCH - 1)) {
+          mid();
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
     }
   };

@@ -98,7 +98,10 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   constexpr bool AEXP = sizeof(T) == 4 ? T1M_AEXP32 != 0 : T1M_AEXP64 != 0;
   constexpr bool BSWZ = sizeof(T) == 4 ? T1M_BSWZ32 != 0 : T1M_BSWZ64 != 0;
   constexpr bool FINE = T1M_FINE != 0;
-  constexpr bool MIDW = FINE ? T1M_FINE_MIDW != 0 : (T1M_MIDW != 0 && BM * BN < 144 * 144);
+  // (64-bit offsets on the 144 x 144 tiles leave no registers for the
+  // mid-stage writes.)
+  constexpr bool MIDW = FINE ? (T1M_FINE_MIDW != 0 && (sizeof(Idx) == 4 || BM * BN < 144 * 144))
+                             : (T1M_MIDW != 0 && BM * BN < 144 * 144);
   // LDS layouts.  A holds each complex element twice, pre-expanded for the
   // two row parities of 1M: (re, -im) for even lanes, (im, re) for odd ones,
   // so the MFMA operands come straight from the LDS reads.  Both tiles are

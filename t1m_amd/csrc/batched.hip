@@ -59,7 +59,7 @@ int cu_count_batch() {
 int batch_kernel(int dtype, int precision) {
   if (const char* f = getenv("T1M_FORCE_KERNEL")) {
     const int k = atoi(f);
-    if (kernel_fits(k, dtype, precision)) return k;
+    if (kernel_fits(k, dtype, precision) && k != K_C64_F32_144 && k != K_C64_BF16X3_144) return k;
   }
   if (precision == T1M_PRECISION_SIMPLE) return K_SIMPLE;
   switch (dtype) {

@@ -574,7 +574,7 @@ struct Batched1M {
 template <typename Idx, typename F>
 auto with_1m_config(int32_t kernel, F&& f) {
   switch (kernel) {
-    case K_C64_F32_144: return f(Batched1M<float, Idx, 144, 144, T1M_BK144, 2, 4>());
+    case K_C64_F32_144:   // batches run the small tiles only
     case K_C64_F32_64: return f(Batched1M<float, Idx, 64, 64, 16, 2, 2>());
     case K_C128_F64_64: return f(Batched1M<double, Idx, 64, 64, 16, 2, 2>());
     default: return f(Batched1M<double, Idx, 32, 32, 16, 2, 2>());

@@ -90,9 +90,12 @@ void t1m_options_init(t1m_options* opt);
 /*
  * C += alpha * A . B over einsum-style labels (one character per mode).
  * Replaces t1m::internal::contract (t1m.h:432-557) as called by both public
- * overloads (t1m.h:560-577).  Labels shared by A and B are contracted; every
- * other label of A and of B must appear in C, and C has no other labels.
- * Unlike the reference, malformed labels and mismatched extents are reported
+ * overloads (t1m.h:560-577).  Labels shared by A and B only are contracted;
+ * every other label of A and of B must appear in C, and C has no other
+ * labels.  A label in A, B and C is a batch mode, C[..z..] += alpha *
+ * sum_k A[..z..] B[..z..] (einsum semantics; the reference sums over such a
+ * label and leaves C's mode unindexed, t1m.h:106-119).  Unlike the
+ * reference, malformed labels and mismatched extents are reported
  * (T1M_ERROR_INVALID_ARGUMENT) instead of being undefined behaviour.
  */
 int t1m_contract(const t1m_tensor* a, const char* la, const t1m_tensor* b,

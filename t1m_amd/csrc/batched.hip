@@ -142,7 +142,7 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
     if ((rc = parse_problem(a + i, la, b + i, lb, c + i, lc, P))) return rc;
     if (i > 0 && P.dtype != a[0].dtype)
       return set_error(T1M_ERROR_INVALID_ARGUMENT, "batch instances must share one dtype");
-    if (P.M == 0 || P.N == 0 || P.K == 0) continue;
+    if (P.M == 0 || P.N == 0 || P.K == 0 || P.L == 0) continue;
     if (!is_device_pointer(a[i].data) || !is_device_pointer(b[i].data) ||
         !is_device_pointer(c[i].data))
       return set_error(T1M_ERROR_INVALID_ARGUMENT,
@@ -161,8 +161,17 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
     g.idx32 = plan->idx32; g.dtype = P.dtype;
     g.split_k = 1;   // instances run side by side; no atomics needed
     g.cus = 0;
-    bt->plans.push_back(plan);
-    bt->problems.push_back(g);
+    // An instance with batch labels contributes one problem per batch index.
+    for (int64_t z = 0; z < plan->L; ++z) {
+      GemmProblem gz = g;
+      if (plan->L > 1) {
+        gz.A = static_cast<const char*>(g.A) + plan->hboff[size_t(z)];
+        gz.B = static_cast<const char*>(g.B) + plan->hboff[size_t(plan->L + z)];
+        gz.C = static_cast<char*>(g.C) + plan->hboff[size_t(2 * plan->L + z)];
+      }
+      bt->plans.push_back(plan);
+      bt->problems.push_back(gz);
+    }
   }
   if ((rc = build_persistent(*bt, o.precision))) return rc;
   if (!bt->persistent && o.autotune) {

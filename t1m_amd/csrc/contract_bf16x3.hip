@@ -464,17 +464,19 @@ k_contract_bf16x3_batched(const KArgs<Idx>* __restrict__ inst,
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN>
 hipError_t launch_t(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
-  const dim3 grid(unsigned(a.tiles_m * a.tiles_n * a.ksplit)), block(64 * WGM * WGN);
+  const int64_t per = int64_t(a.tiles_m) * a.tiles_n * a.ksplit;
+  const dim3 block(64 * WGM * WGN);
   const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
-  if (af && bf)
-    hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, a);
-  else if (af)
-    hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, true, false>), grid, block, 0, s, a);
-  else if (bf)
-    hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, false, true>), grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, false, false>), grid, block, 0, s, a);
-  return hipGetLastError();
+  return launch_batches(a, per, 64 * WGM * WGN, [&](const KArgs<Idx>& q, unsigned grid) {
+    if (af && bf)
+      hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, q);
+    else if (af)
+      hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, true, false>), grid, block, 0, s, q);
+    else if (bf)
+      hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, false, true>), grid, block, 0, s, q);
+    else
+      hipLaunchKernelGGL((k_contract_bf16x3<Idx, BM, BN, BK, WGM, WGN, false, false>), grid, block, 0, s, q);
+  });
 }
 
 template <typename Idx>

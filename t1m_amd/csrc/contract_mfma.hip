@@ -443,7 +443,10 @@ k_contract_1m_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __re
 // MFMA tiles do not cover (real dtypes).
 template <typename T, bool kComplex, typename Idx>
 __global__ void __launch_bounds__(256) k_contract_simple(KArgs<Idx> p) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t per = (p.M * p.N + 255) / 256;       // blocks per batch problem
+  const int z = int(int64_t(blockIdx.x) / per);
+  batch_shift(p, z);
+  const int64_t e = (int64_t(blockIdx.x) - z * per) * blockDim.x + threadIdx.x;
   if (e >= p.M * p.N) return;
   const int64_t i = e % p.M, j = e / p.M;
   const Idx ra = p.rsA[i], cb = p.csB[j];
@@ -479,30 +482,32 @@ template <typename Idx>
 hipError_t launch_simple(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, 0, 0);
   const int64_t n = g.M * g.N;
-  const dim3 grid(unsigned((n + 255) / 256)), block(256);
-  switch (g.dtype) {
-    case T1M_FLOAT32: hipLaunchKernelGGL((k_contract_simple<float, false, Idx>), grid, block, 0, s, a); break;
-    case T1M_FLOAT64: hipLaunchKernelGGL((k_contract_simple<double, false, Idx>), grid, block, 0, s, a); break;
-    case T1M_COMPLEX64: hipLaunchKernelGGL((k_contract_simple<float, true, Idx>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_contract_simple<double, true, Idx>), grid, block, 0, s, a); break;
-  }
-  return hipGetLastError();
+  return launch_batches(a, (n + 255) / 256, 256, [&](const KArgs<Idx>& q, unsigned grid) {
+    switch (g.dtype) {
+      case T1M_FLOAT32: hipLaunchKernelGGL((k_contract_simple<float, false, Idx>), grid, 256, 0, s, q); break;
+      case T1M_FLOAT64: hipLaunchKernelGGL((k_contract_simple<double, false, Idx>), grid, 256, 0, s, q); break;
+      case T1M_COMPLEX64: hipLaunchKernelGGL((k_contract_simple<float, true, Idx>), grid, 256, 0, s, q); break;
+      default: hipLaunchKernelGGL((k_contract_simple<double, true, Idx>), grid, 256, 0, s, q); break;
+    }
+  });
 }
 
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN>
 hipError_t launch_1m(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
-  const dim3 grid(unsigned(a.tiles_m * a.tiles_n * a.ksplit)), block(64 * WGM * WGN);
+  const int64_t per = int64_t(a.tiles_m) * a.tiles_n * a.ksplit;
+  const dim3 block(64 * WGM * WGN);
   const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
-  if (af && bf)
-    hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, a);
-  else if (af)
-    hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, false>), grid, block, 0, s, a);
-  else if (bf)
-    hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, true>), grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, false>), grid, block, 0, s, a);
-  return hipGetLastError();
+  return launch_batches(a, per, 64 * WGM * WGN, [&](const KArgs<Idx>& q, unsigned grid) {
+    if (af && bf)
+      hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, q);
+    else if (af)
+      hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, false>), grid, block, 0, s, q);
+    else if (bf)
+      hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, true>), grid, block, 0, s, q);
+    else
+      hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, false>), grid, block, 0, s, q);
+  });
 }
 
 template <typename Idx>

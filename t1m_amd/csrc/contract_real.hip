@@ -203,9 +203,11 @@ struct RealCfg {
   using KB = void (*)(const KArgs<Idx>*, const BatchItem*, int);
   template <bool AF, bool BF>
   static hipError_t launch_one(const KArgs<Idx>& a, hipStream_t s) {
-    const dim3 grid(unsigned(a.tiles_m * a.tiles_n * a.ksplit)), block(64 * WGM * WGN);
-    hipLaunchKernelGGL((k_contract_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>), grid, block, 0, s, a);
-    return hipGetLastError();
+    const int64_t per = int64_t(a.tiles_m) * a.tiles_n * a.ksplit;
+    return launch_batches(a, per, 64 * WGM * WGN, [&](const KArgs<Idx>& q, unsigned grid) {
+      hipLaunchKernelGGL((k_contract_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>), grid,
+                         dim3(64 * WGM * WGN), 0, s, q);
+    });
   }
   static hipError_t launch(const GemmProblem& g, hipStream_t s) {
     const KArgs<Idx> a = make_args<Idx>(g, BM, BN);

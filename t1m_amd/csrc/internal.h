@@ -62,6 +62,12 @@ struct GemmProblem {
   int32_t dtype;     // t1m_dtype
   int32_t split_k;   // 0: split K when the tiles cannot fill the CUs; 1: never
   int32_t cus;       // compute units of the device (split-K sizing)
+  // Batch modes (labels in A, B and C): L independent problems, problem z
+  // reading A, B, C at byte offsets boffA[z], boffB[z], boffC[z] (device).
+  int64_t L = 1;
+  const int64_t* boffA = nullptr;
+  const int64_t* boffB = nullptr;
+  const int64_t* boffC = nullptr;
 };
 
 // Kernel families.

@@ -37,7 +37,21 @@ struct KArgs {
   int32_t tiles_m, tiles_n;
   int32_t ksplit;    // workgroups per tile along K (1: no split)
   int32_t kchunk;    // contracted extent per split, a multiple of the stage depth
+  int32_t nbatch;    // batch problems in this launch (1: none)
+  const int64_t* boffA;   // byte offsets of the batch problems (nbatch > 1)
+  const int64_t* boffB;
+  const int64_t* boffC;
 };
+
+// Moves the operand pointers of q to batch problem z (of this launch).
+template <typename Idx>
+__device__ inline void batch_shift(KArgs<Idx>& q, int z) {
+  if (q.boffA) {
+    q.A = static_cast<const char*>(q.A) + q.boffA[z];
+    q.B = static_cast<const char*>(q.B) + q.boffB[z];
+    q.C = static_cast<char*>(q.C) + q.boffC[z];
+  }
+}
 
 // C[idx] += v; atomically when several workgroups share the tile (split K).
 template <typename T>
@@ -155,13 +169,20 @@ __device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
   tile_coords_of(int(blockIdx.x), tiles_m, tiles_n, tm, tn);
 }
 
-// Tile and K range of the workgroup: the grid is tiles x ksplit, tile-major.
+// Tile, K range and batch problem of the workgroup: the grid is
+// tiles x ksplit x nbatch, tile-major; q's operands are moved to the batch
+// problem.
 template <typename Idx>
-__device__ inline void tile_and_krange(const KArgs<Idx>& p, int& tm, int& tn, int& kbeg,
+__device__ inline void tile_and_krange(KArgs<Idx>& q, int& tm, int& tn, int& kbeg,
                                        int& kend) {
+  const KArgs<Idx>& p = q;
   const int ntiles = p.tiles_m * p.tiles_n;
-  const int w = int(blockIdx.x) % ntiles;
-  const int split = int(blockIdx.x) / ntiles;
+  const int per = ntiles * p.ksplit;
+  const int z = int(blockIdx.x) / per;
+  const int r = int(blockIdx.x) - z * per;
+  batch_shift(q, z);
+  const int w = r % ntiles;
+  const int split = r / ntiles;
   tile_coords_of(w, p.tiles_m, p.tiles_n, tm, tn);
   kbeg = split * p.kchunk;
   const int64_t e = int64_t(kbeg) + p.kchunk;
@@ -190,7 +211,11 @@ KArgs<Idx> make_args(const GemmProblem& g, int bm, int bn, int bk = 16) {
   a.tiles_n = bn > 0 ? int((g.N + bn - 1) / bn) : 0;
   a.ksplit = 1;
   a.kchunk = int(g.K);
-  const int64_t tiles = int64_t(a.tiles_m) * a.tiles_n;
+  a.nbatch = int32_t(g.L > 0 ? g.L : 1);
+  a.boffA = g.L > 1 ? g.boffA : nullptr;
+  a.boffB = g.L > 1 ? g.boffB : nullptr;
+  a.boffC = g.L > 1 ? g.boffC : nullptr;
+  const int64_t tiles = int64_t(a.tiles_m) * a.tiles_n * (g.L > 0 ? g.L : 1);
   const int cus = g.cus > 0 ? g.cus : 256;
   if (g.split_k == 0 && bm > 0 && tiles > 0 && 2 * tiles < cus) {
     const int64_t want = (2 * int64_t(cus) + tiles - 1) / tiles;     // ~2 waves of workgroups
@@ -203,6 +228,28 @@ KArgs<Idx> make_args(const GemmProblem& g, int bm, int bn, int bk = 16) {
     }
   }
   return a;
+}
+
+// Runs launch(q, grid) over blocks_per_problem x nbatch workgroups of
+// `threads` lanes, in several launches when one grid would pass 2^31 lanes
+// (each launch sees its batch problems as 0.. with the offset tables moved).
+template <typename Idx, typename F>
+hipError_t launch_batches(const KArgs<Idx>& a, int64_t blocks_per_problem, int threads, F&& launch) {
+  const int64_t limit = ((int64_t(1) << 31) - 1) / threads;
+  if (blocks_per_problem <= 0) return hipSuccess;
+  if (blocks_per_problem > limit) return hipErrorInvalidConfiguration;
+  const int64_t L = a.nbatch > 0 ? a.nbatch : 1;
+  const int64_t zmax = limit / blocks_per_problem;
+  for (int64_t z0 = 0; z0 < L; z0 += zmax) {
+    KArgs<Idx> q = a;
+    const int64_t nz = L - z0 < zmax ? L - z0 : zmax;
+    q.nbatch = int32_t(nz);
+    if (q.boffA) { q.boffA += z0; q.boffB += z0; q.boffC += z0; }
+    launch(q, unsigned(blocks_per_problem * nz));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace t1m_hip

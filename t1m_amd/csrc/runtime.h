@@ -39,7 +39,8 @@ struct Problem {
   int dtype = 0;
   std::string la, lb, lc;
   std::vector<Mode> m, n, k;
-  int64_t M = 1, N = 1, K = 1;
+  std::vector<Mode> batch;          // labels in A, B and C (einsum batch modes)
+  int64_t M = 1, N = 1, K = 1, L = 1;
   std::vector<int64_t> da, sa, db, sb, dc, sc;
 };
 
@@ -51,9 +52,12 @@ struct Plan {
   std::atomic<bool> tuned{false};
   std::mutex tune_mu;
   int64_t c_lo = 0, c_hi = 0;                    // C's element span
-  int64_t M = 0, N = 0, K = 0;
+  int64_t M = 0, N = 0, K = 0, L = 1;
   int a_mfast = 1, b_kfast = 1, idx32 = 1;
   void* maps = nullptr;
+  // Batch problems (L > 1): byte offsets into A, B, C, device [3][L] and host.
+  int64_t* boff = nullptr;
+  std::vector<int64_t> hboff;
   const void *rsA = nullptr, *rsC = nullptr, *csA = nullptr, *rsB = nullptr,
              *csB = nullptr, *csC = nullptr;
   std::vector<Mode> modes_m, modes_n, modes_k;   // kernel traversal order

@@ -109,7 +109,7 @@ int parse_problem(const t1m_tensor* a, const char* la, const t1m_tensor* b,
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "A, B and C must share one dtype");
   P.dtype = a->dtype;
   P.la = la; P.lb = lb; P.lc = lc;
-  P.m.clear(); P.n.clear(); P.k.clear();
+  P.m.clear(); P.n.clear(); P.k.clear(); P.batch.clear();
   auto pos = [](const char* s, char ch) -> int {
     const char* q = strchr(s, ch);
     return (q && ch) ? int(q - s) : -1;
@@ -120,10 +120,14 @@ int parse_problem(const t1m_tensor* a, const char* la, const t1m_tensor* b,
   std::sort(sorted_b.begin(), sorted_b.end());
   for (char ch : sorted_a) {
     const int pa = pos(la, ch), pb = pos(lb, ch), pc = pos(lc, ch);
-    if (pb >= 0) {
-      if (pc >= 0)
-        return set_error(T1M_ERROR_UNSUPPORTED,
-                         "label '%c' appears in A, B and C (batch labels are not supported)", ch);
+    if (pb >= 0 && pc >= 0) {
+      // In A, B and C: a batch mode, C[..z..] = sum_k A[..z..] B[..z..]
+      // (einsum semantics; the reference would sum over it, t1m.h:106-119).
+      if (a->dims[pa] != b->dims[pb] || a->dims[pa] != c->dims[pc])
+        return set_error(T1M_ERROR_INVALID_ARGUMENT,
+                         "extent mismatch for batch label '%c'", ch);
+      P.batch.push_back({ch, a->dims[pa], sa[pa], sb[pb], sc[pc]});
+    } else if (pb >= 0) {
       if (a->dims[pa] != b->dims[pb])
         return set_error(T1M_ERROR_INVALID_ARGUMENT,
                          "extent mismatch for contracted label '%c': %lld vs %lld", ch,
@@ -154,10 +158,13 @@ int parse_problem(const t1m_tensor* a, const char* la, const t1m_tensor* b,
     if (pos(la, *q) < 0 && pos(lb, *q) < 0)
       return set_error(T1M_ERROR_INVALID_ARGUMENT,
                        "label '%c' of C is in neither A nor B", *q);
-  P.M = P.N = P.K = 1;
+  P.M = P.N = P.K = P.L = 1;
   for (auto& x : P.m) P.M *= x.dim;
   for (auto& x : P.n) P.N *= x.dim;
   for (auto& x : P.k) P.K *= x.dim;
+  for (auto& x : P.batch) P.L *= x.dim;
+  if (P.L > INT32_MAX)
+    return set_error(T1M_ERROR_UNSUPPORTED, "more than 2^31 - 1 batch problems");
   P.sa = sa; P.sb = sb; P.sc = sc;
   P.da.assign(a->dims, a->dims + a->rank);
   P.db.assign(b->dims, b->dims + b->rank);
@@ -180,16 +187,6 @@ int cu_count() {
     return cus > 0 ? cus : 256;
   }();
   return n;
-}
-
-// Relative cost of running an M x N (x K) problem with BM x BN tiles:
-// rounds of workgroups times per-tile work over the tile's expected MFMA
-// efficiency.  Resident workgroups per CU = occ.
-double tile_cost(int64_t M, int64_t N, int bm, int bn, int occ, double eff) {
-  const int64_t tiles = ((M + bm - 1) / bm) * ((N + bn - 1) / bn);
-  const int64_t slots = int64_t(cu_count()) * occ;
-  const int64_t rounds = (tiles + slots - 1) / slots;
-  return double(rounds) * double(bm) * double(bn) / (eff * occ);
 }
 
 }  // namespace
@@ -230,7 +227,7 @@ int choose_kernel(const Problem& P, int precision) {
   }
   if (precision == T1M_PRECISION_SIMPLE) return K_SIMPLE;
   auto tiles = [&](int bm, int bn) {
-    return double((P.M + bm - 1) / bm) * double((P.N + bn - 1) / bn);
+    return double((P.M + bm - 1) / bm) * double((P.N + bn - 1) / bn) * double(P.L);
   };
   switch (P.dtype) {
     case T1M_COMPLEX64: {
@@ -316,6 +313,10 @@ std::string tune_key(const Plan& pl) {
   snprintf(buf, sizeof(buf), "%s/%d|dt%d|p%d|M%lld|N%lld|K%lld|a%d|b%d|i%d", arch.c_str(),
            cu_count(), pl.dtype, pl.precision, (long long)pl.M, (long long)pl.N,
            (long long)pl.K, pl.a_mfast, pl.b_kfast, pl.idx32);
+  if (pl.L > 1) {
+    const size_t n = strlen(buf);
+    snprintf(buf + n, sizeof(buf) - n, "|L%lld", (long long)pl.L);
+  }
   return buf;
 }
 
@@ -546,6 +547,33 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
     return set_error(T1M_ERROR_DEVICE, "index map construction failed: %s", hipGetErrorString(e));
   }
   pl.modes_m = P.m; pl.modes_n = P.n; pl.modes_k = P.k;
+  pl.L = P.L;
+  if (P.L > 1) {
+    // Byte offsets of batch problem z (mixed radix over the batch modes,
+    // first mode fastest).
+    const int64_t es = dtype_size(P.dtype);
+    pl.hboff.assign(size_t(3 * P.L), 0);
+    for (int64_t z = 0; z < P.L; ++z) {
+      int64_t r = z, oa = 0, ob = 0, oc = 0;
+      for (const Mode& md : P.batch) {
+        const int64_t d = r % md.dim;
+        r /= md.dim;
+        oa += d * md.sa; ob += d * md.sb; oc += d * md.sc;
+      }
+      pl.hboff[size_t(z)] = oa * es;
+      pl.hboff[size_t(P.L + z)] = ob * es;
+      pl.hboff[size_t(2 * P.L + z)] = oc * es;
+    }
+    const size_t bytes = pl.hboff.size() * sizeof(int64_t);
+    if (hipMalloc(reinterpret_cast<void**>(&pl.boff), bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      pl.boff = nullptr;
+      return set_error(T1M_ERROR_OUT_OF_MEMORY, "hipMalloc of %zu bytes for batch offsets failed", bytes);
+    }
+    hipError_t eb = hipMemcpy(pl.boff, pl.hboff.data(), bytes, hipMemcpyHostToDevice);
+    if (eb != hipSuccess)
+      return set_error(T1M_ERROR_DEVICE, "batch offsets upload failed: %s", hipGetErrorString(eb));
+  }
   if (!getenv("T1M_FORCE_KERNEL")) {     // a tuning result from an earlier run
     TuneCache& T = tune_cache();
     std::lock_guard<std::mutex> lock(T.mu);
@@ -575,6 +603,7 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
 
 Plan::~Plan() {
   if (maps) (void)hipFree(maps);
+  if (boff) (void)hipFree(boff);
 }
 
 int64_t plan_cache_size() {
@@ -653,7 +682,7 @@ int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "unknown precision %d", opt.precision);
   if (opt.precision == T1M_PRECISION_BF16X3 && P.dtype != T1M_COMPLEX64)
     return set_error(T1M_ERROR_UNSUPPORTED, "BF16x3 needs complex<float> data");
-  if (P.M == 0 || P.N == 0 || P.K == 0) return T1M_SUCCESS;   // C += 0
+  if (P.M == 0 || P.N == 0 || P.K == 0 || P.L == 0) return T1M_SUCCESS;   // C += 0
   if (!a->data || !b->data || !c->data)
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "null data pointer");
   hipStream_t stream = static_cast<hipStream_t>(opt.stream);
@@ -687,6 +716,12 @@ int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
   g.dtype = P.dtype;
   g.split_k = opt.deterministic ? 1 : 0;
   g.cus = cu_count();
+  g.L = plan->L;
+  if (plan->L > 1) {
+    g.boffA = plan->boff;
+    g.boffB = plan->boff + plan->L;
+    g.boffC = plan->boff + 2 * plan->L;
+  }
   if (opt.autotune && !plan->tuned.load() && (rc = autotune_plan(*plan, g, stream))) return rc;
   hipError_t e = launch_contract(g, plan->kernel, stream);
   if (e != hipSuccess)
@@ -740,7 +775,7 @@ const char* t1m_plan_kernel(const t1m_tensor* a, const char* la, const t1m_tenso
   if (opt) o = *opt; else t1m_options_init(&o);
   Problem P;
   if (parse_problem(a, la, b, lb, c, lc, P)) return nullptr;
-  if (P.M == 0 || P.N == 0 || P.K == 0) return "none";
+  if (P.M == 0 || P.N == 0 || P.K == 0 || P.L == 0) return "none";
   std::shared_ptr<Plan> plan;
   if (get_plan(P, o.precision, static_cast<hipStream_t>(o.stream), plan)) return nullptr;
   return kernel_info(plan->kernel.load()).name;

@@ -64,7 +64,7 @@ def _t(shape, dtype=torch.complex64):
 
 @pytest.mark.parametrize("labels,shapes,status", [
     (("aa", "bc", "ac"), ((2, 2), (3, 4), (2, 4)), 1),     # repeated label
-    (("ab", "bc", "ab"), ((2, 3), (3, 4), (2, 3)), 4),     # batch label a,b,c
+    (("ab", "bc", "abc"), ((2, 3), (4, 5), (2, 3, 5)), 1), # batch label b: extents
     (("ab", "cd", "ac"), ((2, 3), (3, 4), (2, 3)), 1),     # b of A nowhere
     (("ab", "bc", "ad"), ((2, 3), (3, 4), (2, 4)), 1),     # d of C nowhere
     (("ab", "bc", "ac"), ((2, 3), (5, 4), (2, 4)), 1),     # contracted extent

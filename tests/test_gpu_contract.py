@@ -287,6 +287,84 @@ def test_split_k_small_output_long_k(dtype, kern, prec, tolkey):
     np.testing.assert_array_equal(d1, d2)
 
 
+# Batch modes: labels in A, B and C, C[..z..] = sum_k A[..z..] B[..z..]
+# (einsum semantics; the reference leaves such labels undefined).
+BATCH_CASES = [
+    (("zab", "zbc", "zac"), {"z": 5, "a": 37, "b": 19, "c": 29}),            # batched GEMM
+    (("abz", "bcz", "acz"), {"z": 3, "a": 70, "b": 33, "c": 41}),            # batch mode slowest
+    (("azbw", "wbcz", "cazw"), {"z": 2, "w": 3, "a": 23, "b": 9, "c": 17}),  # two, interleaved
+    (("ab", "ab", "ab"), {"a": 31, "b": 7}),                                 # Hadamard product
+    (("abk", "bk", "ab"), {"a": 20, "b": 6, "k": 5}),                        # batch, no free B mode
+    (("zab", "zbc", "zac"), {"z": 2, "a": 40, "b": 2000, "c": 50}),          # batch + split K
+]
+REAL_BATCH = {np.float32: [(k, "native", "fp32") for k in REAL_KERNELS[np.float32]],
+              np.float64: [(k, "native", "fp64") for k in REAL_KERNELS[np.float64]]}
+
+
+@pytest.mark.parametrize("case", range(len(BATCH_CASES)))
+def test_batch_labels_all_kernels(case):
+    (la, lb, lc), ext = BATCH_CASES[case]
+    rng = np.random.default_rng(40 + case)
+    for dtype, kernels in ((np.complex64, KERNELS_C64), (np.complex128, KERNELS_C128),
+                           (np.float32, REAL_BATCH[np.float32]),
+                           (np.float64, REAL_BATCH[np.float64])):
+        A = rand(rng, [ext[x] for x in la], dtype)
+        B = rand(rng, [ext[x] for x in lb], dtype)
+        C0 = rand(rng, [ext[x] for x in lc], dtype)
+        alpha = 0.75 if np.dtype(dtype).kind == "f" else 0.5 - 0.25j
+        want = oracle_einsum(A, la, B, lb, C0, lc, alpha=alpha)
+        exact = np.einsum(f"{la},{lb}->{lc}", A.astype(np.complex128), B.astype(np.complex128))
+        assert rel_err(want - C0, alpha * exact) < 1e-6          # oracle vs numpy
+        for kern, prec, tolkey in kernels:
+            with forced(kern):
+                got = run(A, la, B, lb, C0, lc, precision=prec, alpha=alpha)
+            err = rel_err(got - C0, want - C0)
+            print(f"  {la},{lb}->{lc} {np.dtype(dtype).name} {kern} rel={err:.2e}")
+            assert err < TOL[tolkey], (kern, err)
+
+
+def test_batch_labels_empty_batch_and_vjp():
+    rng = np.random.default_rng(47)
+    A = rand(rng, (0, 5, 6), np.complex64)          # z = 0: nothing to add
+    B = rand(rng, (0, 6, 7), np.complex64)
+    C0 = rand(rng, (0, 5, 7), np.complex64)
+    assert run(A, "zab", B, "zbc", C0, "zac").shape == (0, 5, 7)
+    la, lb, lc = "azb", "bcz", "acz"
+    ext = {"z": 3, "a": 6, "b": 5, "c": 7}
+    A = rand(rng, [ext[x] for x in la], np.complex128)
+    B = rand(rng, [ext[x] for x in lb], np.complex128)
+    G = rand(rng, [ext[x] for x in lc], np.complex128)
+    ga_want = oracle_einsum(G, lc, B, lb, np.zeros_like(A, order="F"), la, conj_b=True)
+    gb_want = oracle_einsum(A, la, G, lc, np.zeros_like(B, order="F"), lb, conj_a=True)
+    ga = to_dev(np.zeros_like(A, order="F"))
+    gb = to_dev(np.zeros_like(B, order="F"))
+    t1m_amd.contract_vjp(to_dev(A), la, to_dev(B), lb, to_dev(G), lc, ga, gb)
+    assert rel_err(to_host(ga), ga_want) < 1e-13
+    assert rel_err(to_host(gb), gb_want) < 1e-13
+
+
+def test_batch_labels_in_persistent_batch():
+    """Instances with batch labels are flattened into one problem per batch
+    index of the persistent launch."""
+    rng = np.random.default_rng(48)
+    la, lb, lc = "zasb", "zbtc", "zastc"
+    As, Bs, Cs, wants = [], [], [], []
+    for _ in range(5):
+        z, dl, db, dr = (int(x) for x in rng.choice([2, 3, 16, 24, 40], 4))
+        A = rand(rng, (z, dl, 2, db), np.complex64)
+        B = rand(rng, (z, db, 2, dr), np.complex64)
+        C0 = rand(rng, (z, dl, 2, 2, dr), np.complex64)
+        wants.append(oracle_einsum(A, la, B, lb, C0, lc) - C0)
+        As.append(to_dev(A)); Bs.append(to_dev(B)); Cs.append(to_dev(C0))
+        Cs[-1]._c0 = C0
+    batch = t1m_amd.Batch(As, la, Bs, lb, Cs, lc)
+    batch.launch()
+    torch.cuda.synchronize()
+    for c, w in zip(Cs, wants):
+        assert rel_err(to_host(c) - c._c0, w) < TOL["fp32"]
+    batch.close()
+
+
 def test_conj_alpha_and_accumulate():
     rng = np.random.default_rng(8)
     la, lb, lc = "abcd", "dbef", "acef"

@@ -324,3 +324,21 @@ def test_einsum_conj_and_alpha():
         for j in range(5):
             s = sum(A[i, p] * B[p, j] for p in range(4))
             assert abs(C[i, j] - s) < 1e-13
+
+
+@pytest.mark.parametrize("labels", [("zab", "zbc", "zac"), ("azbw", "wbcz", "cazw"),
+                                    ("ab", "ab", "ab"), ("abk", "bk", "ab")])
+def test_einsum_batch_labels(labels):
+    """Labels in A, B and C are batch modes (einsum semantics): the oracle's
+    label loops against numpy.einsum, and a brute-force loop for the first."""
+    la, lb, lc = labels
+    rng = np.random.default_rng(11)
+    ext = {ch: int(rng.integers(2, 5)) for ch in set(la + lb + lc)}
+    A = _rand(rng, [ext[x] for x in la], np.complex128)
+    B = _rand(rng, [ext[x] for x in lb], np.complex128)
+    C = np.zeros([ext[x] for x in lc], dtype=np.complex128, order="F")
+    orc.einsum(A, la, B, lb, C, lc)
+    np.testing.assert_allclose(C, np.einsum(f"{la},{lb}->{lc}", A, B), rtol=1e-13, atol=1e-13)
+    if labels[0] == "zab":
+        for z in range(ext["z"]):
+            np.testing.assert_allclose(C[z], A[z] @ B[z], rtol=1e-13)

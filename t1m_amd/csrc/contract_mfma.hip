@@ -76,6 +76,18 @@
 #ifndef T1M_FINE_MIDW
 #define T1M_FINE_MIDW 1
 #endif
+// Wave layout of the FP64 64 x 64 tiles.
+#ifndef T1M_F64_WGM
+#define T1M_F64_WGM 2
+#endif
+#ifndef T1M_F64_WGN
+#define T1M_F64_WGN 2
+#endif
+// Wave priority (s_setprio) around the MFMA stream: 0 none, 1 raised while
+// issuing each MFMA group, 2 raised for the whole stage except the writes.
+#ifndef T1M_PRIO
+#define T1M_PRIO 2
+#endif
 
 namespace t1m_hip {
 
@@ -311,6 +323,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   auto compute_fine = [&](int buf, auto&& mid) {
     BFr fb[2];
     C2 aw[LA + 1];
+    if (T1M_PRIO == 2) __builtin_amdgcn_s_setprio(1);
     load_b(buf, 0, fb[0]);
 #pragma unroll
     for (int t = 0; t < LA; ++t) aw[t] = load_a(buf, t);
@@ -320,13 +333,18 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
       if (grp == 0 && kc + 1 < KCH) load_b(buf, kc + 1, fb[(kc + 1) & 1]);
       if (t + LA < TOT) aw[(t + LA) % (LA + 1)] = load_a(buf, t + LA);
       __builtin_amdgcn_sched_barrier(0);
+      if (T1M_PRIO == 1) __builtin_amdgcn_s_setprio(1);
       mma_grp(grp, aw[t % (LA + 1)], fb[kc & 1]);
+      if (T1M_PRIO == 1) __builtin_amdgcn_s_setprio(0);
       __builtin_amdgcn_sched_barrier(0);
       if (MIDW && t == TOT / 2 - 1) {
+        if (T1M_PRIO == 2) __builtin_amdgcn_s_setprio(0);
         mid();
+        if (T1M_PRIO == 2) __builtin_amdgcn_s_setprio(1);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
+    if (T1M_PRIO == 2) __builtin_amdgcn_s_setprio(0);
     if (!MIDW) mid();
   };
   auto compute = [&](int buf, auto&& mid) {
@@ -515,7 +533,7 @@ hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
   switch (kernel) {
     case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, T1M_BK144, 2, 4>(g, s);
     case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 16, 2, 2>(g, s);
-    case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16, 2, 2>(g, s);
+    case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16, T1M_F64_WGM, T1M_F64_WGN>(g, s);
     case K_C128_F64_32: return launch_1m<double, Idx, 32, 32, 16, 2, 2>(g, s);
     default: return launch_simple<Idx>(g, s);
   }

@@ -117,53 +117,72 @@ def version() -> str:
     return load().t1m_version().decode()
 
 
+_DTYPE_CODES = None
+
+
 def _dtype_code(t) -> int:
-    import torch
-    codes = {torch.float32: FLOAT32, torch.float64: FLOAT64,
-             torch.complex64: COMPLEX64, torch.complex128: COMPLEX128}
-    if t.dtype not in codes:
+    global _DTYPE_CODES
+    if _DTYPE_CODES is None:
+        import torch
+        _DTYPE_CODES = {torch.float32: FLOAT32, torch.float64: FLOAT64,
+                        torch.complex64: COMPLEX64, torch.complex128: COMPLEX128}
+    code = _DTYPE_CODES.get(t.dtype)
+    if code is None:
         raise TypeError(f"unsupported dtype {t.dtype}")
-    return codes[t.dtype]
+    return code
 
 
 class _Desc:
     """Keeps the ctypes arrays of one tensor descriptor alive."""
 
+    __slots__ = ("dims", "strides", "desc")
+
     def __init__(self, t, dtype: Optional[int] = None, dims=None, strides=None, ptr=None):
-        dims = list(t.shape) if dims is None else list(dims)
-        strides = list(t.stride()) if strides is None else list(strides)
-        self.dims = (ctypes.c_int64 * max(len(dims), 1))(*dims)
-        self.strides = (ctypes.c_int64 * max(len(strides), 1))(*strides)
-        self.desc = TensorDesc(_dtype_code(t) if dtype is None else dtype, len(dims),
+        dims = t.shape if dims is None else dims
+        strides = t.stride() if strides is None else strides
+        n = len(dims)
+        arr = _I64_ARRAYS[n] if n < len(_I64_ARRAYS) else ctypes.c_int64 * max(n, 1)
+        self.dims = arr(*dims)
+        self.strides = arr(*strides)
+        self.desc = TensorDesc(_dtype_code(t) if dtype is None else dtype, n,
                                self.dims, self.strides,
                                t.data_ptr() if ptr is None else ptr)
+
+
+_I64_ARRAYS = [ctypes.c_int64 * max(n, 1) for n in range(33)]
 
 
 def _options(alpha=1.0, precision="native", conj_a=False, conj_b=False,
              stream=None, synchronize=True, autotune=False,
              deterministic=False) -> Options:
-    o = Options()
-    load().t1m_options_init(ctypes.byref(o))
+    # Field by field (the defaults of t1m_options_init), without a library call.
     al = complex(alpha)
-    o.alpha_re, o.alpha_im = al.real, al.imag
-    o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-    o.conj_a, o.conj_b = int(bool(conj_a)), int(bool(conj_b))
-    o.synchronize = int(bool(synchronize))
-    o.stream = _stream_handle(stream)
-    o.autotune = int(bool(autotune))
-    o.deterministic = int(bool(deterministic))
-    return o
+    return Options(al.real, al.imag,
+                   PRECISIONS[precision] if isinstance(precision, str) else int(precision),
+                   int(bool(conj_a)), int(bool(conj_b)), int(bool(synchronize)),
+                   _stream_handle(stream), int(bool(autotune)), int(bool(deterministic)))
+
+
+_RAW_STREAM = None
 
 
 def _stream_handle(stream) -> Optional[int]:
+    global _RAW_STREAM
     if stream is None:
-        try:
-            import torch
-            if torch.cuda.is_available():
-                return torch.cuda.current_stream().cuda_stream or None
-        except ImportError:
-            pass
-        return None
+        if _RAW_STREAM is None:
+            try:
+                import torch
+                if torch.cuda.is_available():
+                    # torch's raw current-stream query (the public accessor
+                    # builds a Python Stream object per call)
+                    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+                    _RAW_STREAM = ((lambda: raw(torch.cuda.current_device())) if raw else
+                                   (lambda: torch.cuda.current_stream().cuda_stream))
+                else:
+                    _RAW_STREAM = lambda: None
+            except ImportError:
+                _RAW_STREAM = lambda: None
+        return _RAW_STREAM() or None
     if isinstance(stream, int):
         return stream or None
     return stream.cuda_stream or None

@@ -76,9 +76,11 @@
 #ifndef T1M_FINE_MIDW
 #define T1M_FINE_MIDW 1
 #endif
-// Wave layout of the FP64 64 x 64 tiles.
+// Wave layout of the FP64 64 x 64 tiles: 4 x 2 waves (16 x 32 complex per
+// wave, 2 waves per SIMD) measured 23 % faster than 2 x 2 at 1024 x 1024 x
+// 256 and 3 % at 2304^3.
 #ifndef T1M_F64_WGM
-#define T1M_F64_WGM 2
+#define T1M_F64_WGM 4
 #endif
 #ifndef T1M_F64_WGN
 #define T1M_F64_WGN 2

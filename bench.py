@@ -109,10 +109,23 @@ def rel_errors(t1m, torch, cfg, seed):
     return out
 
 
-def cpu_baseline(cfg, seconds_hint=15.0):
-    """The oracle's five-loop 1M restatement (single thread) on a bounded
-    slice of the same workload: B's and C's last free mode cut to `cut`
-    values, which scales the flops linearly (8 m n' k)."""
+def cpu_threads():
+    """Host threads for the all-cores baseline: OMP_NUM_THREADS if set, else
+    the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(cfg, seconds_hint=15.0, threads=1):
+    """The oracle's five-loop 1M restatement (one thread: the reference is
+    serial; or loop 3 over `threads` OpenMP threads) on a bounded slice of
+    the same workload: B's and C's last free mode cut to `cut` values, which
+    scales the flops linearly (8 m n' k)."""
     import numpy as np
     from oracle import oracle as orc
     rng = np.random.default_rng(7)
@@ -132,18 +145,18 @@ def cpu_baseline(cfg, seconds_hint=15.0):
         d[lastb] = cut
         A, B, C = arr(cfg["la"], d), arr(cfg["lb"], d), arr(cfg["lc"], d) * 0
         t0 = time.perf_counter()
-        orc.contract(A, cfg["la"], B, cfg["lb"], C, cfg["lc"])
+        orc.contract(A, cfg["la"], B, cfg["lb"], C, cfg["lc"], threads=threads)
         dt_s = time.perf_counter() - t0
         fl = full * cut / dims[lastb]
         if dt_s * 2 > seconds_hint or cut * 2 > dims[lastb]:
             break
         cut *= 2
-    return {"value": round(fl / dt_s / 1e12, 6), "unit": "TFLOP/s", "cores": 1,
+    return {"value": round(fl / dt_s / 1e12, 6), "unit": "TFLOP/s", "cores": threads,
             "kind": "port",
             "sample": (f"oracle five-loop 1M contraction (BLIS-sized blocks, plain C "
                        f"micro-kernel), {cfg['la']},{cfg['lb']}->{cfg['lc']} with "
                        f"'{lastb}' cut to {cut} of {dims[lastb]} ({fl:.3e} flop, "
-                       f"{dt_s:.2f} s, 1 thread)")}
+                       f"{dt_s:.2f} s, {threads} thread{'s' if threads > 1 else ''})")}
 
 
 def _ttgt_views(cfg, a, b):
@@ -378,6 +391,9 @@ def main():
     line["library_gemm"] = library_gemm(cfg, torch, a, b, torch.zeros_like(c), args.steps)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg)
+        nt = cpu_threads()
+        if nt > 1:
+            line["cpu_baseline_all_cores"] = cpu_baseline(cfg, seconds_hint=5.0, threads=nt)
         line["cpu_torch_einsum"] = cpu_einsum(cfg, torch)
     print(json.dumps(line), flush=True)
 

@@ -59,6 +59,8 @@ def lib() -> ctypes.CDLL:
         _lib.orc_unpack.argtypes = [ctypes.c_int] + view + [ctypes.c_void_p, ctypes.c_void_p]
         tens = [ctypes.c_void_p, ctypes.c_int32, u64p, u64p, ctypes.c_char_p]
         _lib.orc_contract.argtypes = ([ctypes.c_int, ctypes.c_double] + tens * 3 + [u64p])
+        _lib.orc_contract_threads.argtypes = ([ctypes.c_int, ctypes.c_double] + tens * 3 +
+                                              [u64p, ctypes.c_int])
         etens = [ctypes.c_void_p, i64p, i64p, ctypes.c_char_p]
         _lib.orc_einsum.argtypes = ([ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_int] + etens * 3)
@@ -152,10 +154,10 @@ def _estrides(arr: np.ndarray) -> list:
 
 
 def contract(a: np.ndarray, la: str, b: np.ndarray, lb: str, c: np.ndarray,
-             lc: str, alpha: float = 1.0, block_sizes=None) -> None:
+             lc: str, alpha: float = 1.0, block_sizes=None, threads: int = 1) -> None:
     """Five-loop 1M/real contraction C += alpha*A*B (t1m.h:432-557) with the
     reference's packing and a plain C micro-kernel.  Arrays may have any
-    non-negative strides."""
+    non-negative strides.  threads > 1 splits loop 3 over OpenMP threads."""
     dt = DTYPES[a.dtype]
     if block_sizes is None:
         block_sizes = BLOCK_SIZES["s" if dt in (0, 2) else "d"]
@@ -167,8 +169,8 @@ def contract(a: np.ndarray, la: str, b: np.ndarray, lb: str, c: np.ndarray,
         keep.extend([d, s])
         return [arr.ctypes.data, arr.ndim, _p64(d), _p64(s)]
 
-    rc = lib().orc_contract(dt, float(alpha), *t(a), la.encode(), *t(b), lb.encode(),
-                            *t(c), lc.encode(), _p64(bs))
+    rc = lib().orc_contract_threads(dt, float(alpha), *t(a), la.encode(), *t(b), lb.encode(),
+                                    *t(c), lc.encode(), _p64(bs), int(threads))
     if rc:
         raise RuntimeError(f"orc_contract failed with {rc}")
 

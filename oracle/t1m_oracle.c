@@ -31,6 +31,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <complex.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORC_MAX_RANK 32
 
@@ -475,12 +478,35 @@ static void ukernel(int dtype, uint64_t m, uint64_t n, uint64_t k, double alpha,
  * Returns 0, or -1 on allocation failure (the reference throws bad_alloc,
  * t1m.h:455-458), or -2 on a bundle error.
  */
+/*
+ * nthreads > 1 runs loop 3 (the MC blocks of A) on that many OpenMP threads,
+ * each with its own A and C buffers; the B panel is packed once per (jc, pc)
+ * as in the serial loop.  Different MC blocks write disjoint rows of C.
+ * The reference is serial; this is the all-cores CPU baseline.
+ */
+int orc_contract_threads(int dtype, double alpha, const void* a, int32_t ra,
+                         const uint64_t* da, const uint64_t* sa, const char* la,
+                         const void* b, int32_t rb, const uint64_t* db,
+                         const uint64_t* sb, const char* lb, void* c, int32_t rc,
+                         const uint64_t* dc, const uint64_t* sc, const char* lc,
+                         const uint64_t* bs, int nthreads);
+
 int orc_contract(int dtype, double alpha, const void* a, int32_t ra,
                  const uint64_t* da, const uint64_t* sa, const char* la,
                  const void* b, int32_t rb, const uint64_t* db,
                  const uint64_t* sb, const char* lb, void* c, int32_t rc,
                  const uint64_t* dc, const uint64_t* sc, const char* lc,
                  const uint64_t* bs) {
+  return orc_contract_threads(dtype, alpha, a, ra, da, sa, la, b, rb, db, sb, lb, c, rc, dc,
+                              sc, lc, bs, 1);
+}
+
+int orc_contract_threads(int dtype, double alpha, const void* a, int32_t ra,
+                         const uint64_t* da, const uint64_t* sa, const char* la,
+                         const void* b, int32_t rb, const uint64_t* db,
+                         const uint64_t* sb, const char* lb, void* c, int32_t rc,
+                         const uint64_t* dc, const uint64_t* sc, const char* lc,
+                         const uint64_t* bs, int nthreads) {
   uint64_t MR = bs[0], NR = bs[1], KP = bs[2], MC = bs[3], KC = bs[4], NC = bs[5];
   uint64_t bund[6 * ORC_MAX_RANK];
   int64_t cnt[6];
@@ -494,10 +520,11 @@ int orc_contract(int dtype, double alpha, const void* a, int32_t ra,
 
   size_t rsz = real_size(dtype);
   uint64_t szA = MC * KC, szB = KC * NC, szC = MR * NR;   /* t1m.h:450-453 */
-  char* space_a = (char*)aligned_alloc(64, ((szA + szB + szC) * rsz + 63) / 64 * 64);
-  if (!space_a) { layout_free(&LA); layout_free(&LB); layout_free(&LC); return -1; }
-  char* space_b = space_a + szA * rsz;
-  char* space_c = space_b + szB * rsz;
+  if (nthreads < 1) nthreads = 1;
+  /* B panel, then one (A block, C tile) pair per thread, 64-byte aligned. */
+  const size_t per = ((szA + szC) * rsz + 63) / 64 * 64, bbytes = (szB * rsz + 63) / 64 * 64;
+  char* space_b = (char*)aligned_alloc(64, bbytes + per * (size_t)nthreads);
+  if (!space_b) { layout_free(&LA); layout_free(&LB); layout_free(&LC); return -1; }
 
   uint64_t M = VA.nrs, K = VA.ncs, N = VB.ncs;              /* t1m.h:462-464 */
   int cplx = dtype >= 2;
@@ -511,7 +538,16 @@ int orc_contract(int dtype, double alpha, const void* a, int32_t ra,
       orc_view vb = subview(VB, pc, jc, k, ncn);
       memset(space_b, 0, szB * rsz);
       pack_block_v(dtype, 0, &vb, KC, b, space_b);
-      for (uint64_t ic = 0; ic < M; ic += mstep) {          /* loop 3 */
+      int64_t nblk = (int64_t)((M + mstep - 1) / mstep);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) if (nthreads > 1)
+      for (int64_t blk = 0; blk < nblk; ++blk) {            /* loop 3 */
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        char* space_a = space_b + bbytes + per * (size_t)tid;
+        char* space_c = space_a + szA * rsz;
+        uint64_t ic = (uint64_t)blk * mstep;
         uint64_t mcm = mstep < M - ic ? mstep : M - ic;
         uint64_t mreal = cplx ? 2 * mcm : mcm;
         orc_view va = subview(VA, ic, pc, mcm, k);
@@ -533,7 +569,7 @@ int orc_contract(int dtype, double alpha, const void* a, int32_t ra,
       }
     }
   }
-  free(space_a);
+  free(space_b);
   layout_free(&LA); layout_free(&LB); layout_free(&LC);
   return 0;
 }

@@ -342,3 +342,20 @@ def test_einsum_batch_labels(labels):
     if labels[0] == "zab":
         for z in range(ext["z"]):
             np.testing.assert_allclose(C[z], A[z] @ B[z], rtol=1e-13)
+
+
+def test_contract_threads_match_serial():
+    """The all-cores variant of the five-loop driver (loop 3 over threads)
+    gives the serial result bit for bit: each C element is accumulated by
+    one thread, in the same order."""
+    rng = np.random.default_rng(21)
+    for dt in (np.complex64, np.float64):
+        A = _rand(rng, (40, 7, 9), dt)
+        B = _rand(rng, (9, 7, 30), dt)
+        C1 = np.zeros((40, 30), dtype=dt, order="F")
+        C4 = np.zeros((40, 30), dtype=dt, order="F")
+        bs = (4, 8, 4, 12, 8, 16)
+        orc.contract(A, "abc", B, "cbd", C1, "ad", block_sizes=bs)
+        orc.contract(A, "abc", B, "cbd", C4, "ad", block_sizes=bs, threads=4)
+        np.testing.assert_array_equal(C1, C4)
+        np.testing.assert_allclose(C1, np.einsum("abc,cbd->ad", A, B), rtol=1e-4)

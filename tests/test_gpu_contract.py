@@ -670,3 +670,68 @@ def test_cpp_dropin_program():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     print(r.stdout[-4000:])
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+# ------------------------------------------------------------ fuzzing ----
+
+ALL_KERNELS = {np.complex64: KERNELS_C64, np.complex128: KERNELS_C128,
+               np.float32: REAL_BATCH[np.float32], np.float64: REAL_BATCH[np.float64]}
+
+
+def _random_problem(rng):
+    """Random labels (contracted, free A, free B, sometimes a batch label)
+    with extents 1..9."""
+    letters = list("abcdefghijklmnop")
+    rng.shuffle(letters)
+    nk, nm, nn, nb = (int(rng.integers(0, 3)), int(rng.integers(0, 3)),
+                      int(rng.integers(0, 3)), int(rng.integers(0, 2)) if rng.random() < 0.3 else 0)
+    k, m, n, z = (letters[:nk], letters[nk:nk + nm], letters[nk + nm:nk + nm + nn],
+                  letters[nk + nm + nn:nk + nm + nn + nb])
+    la = "".join(rng.permutation(k + m + z)) if k + m + z else ""
+    lb = "".join(rng.permutation(k + n + z)) if k + n + z else ""
+    lc = "".join(rng.permutation(m + n + z)) if m + n + z else ""
+    ext = {ch: int(rng.integers(1, 10)) for ch in la + lb + lc}
+    return la, lb, lc, ext
+
+
+def _random_layout(rng, X):
+    """Device tensor holding X's values in a random memory layout: permuted
+    storage order of the modes and padding on one of them."""
+    if X.ndim == 0:
+        return to_dev(X)
+    perm = [int(p) for p in rng.permutation(X.ndim)]
+    shape_p = [X.shape[p] for p in perm]
+    pad = [0] * X.ndim
+    pad[int(rng.integers(0, X.ndim))] = int(rng.integers(0, 3))
+    big = torch.zeros([s_ + q for s_, q in zip(shape_p, pad)],
+                      dtype=torch.from_numpy(np.zeros(1, X.dtype)).dtype, device="cuda")
+    view = big[tuple(slice(0, s_) for s_ in shape_p)]
+    view.copy_(torch.from_numpy(np.ascontiguousarray(X.transpose(perm))))
+    return view.permute(*[int(q) for q in np.argsort(perm)])
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_fuzz_random_contractions(seed):
+    """Random label sets (incl. batch labels, empty bundles, extents of 1),
+    random strides and every kernel of each dtype, against the oracle."""
+    rng = np.random.default_rng(1000 + seed)
+    for case in range(6):
+        la, lb, lc, ext = _random_problem(rng)
+        dtype = [np.complex64, np.complex128, np.float32, np.float64][int(rng.integers(0, 4))]
+        A = rand(rng, [ext[x] for x in la], dtype)
+        B = rand(rng, [ext[x] for x in lb], dtype)
+        C0 = rand(rng, [ext[x] for x in lc], dtype)
+        cplx = np.dtype(dtype).kind == "c"
+        alpha = complex(rng.uniform(-1, 1), rng.uniform(-1, 1)) if cplx else float(rng.uniform(-1, 1))
+        conj_a = cplx and rng.random() < 0.3
+        conj_b = cplx and rng.random() < 0.3
+        want = oracle_einsum(A, la, B, lb, C0, lc, alpha=alpha, conj_a=conj_a, conj_b=conj_b)
+        for kern, prec, tolkey in ALL_KERNELS[dtype]:
+            a, b, c = _random_layout(rng, A), _random_layout(rng, B), _random_layout(rng, C0)
+            with forced(kern):
+                t1m_amd.contract(a, la, b, lb, c, lc, precision=prec, alpha=alpha,
+                                 conj_a=conj_a, conj_b=conj_b)
+            got = c.cpu().numpy()
+            err = rel_err(got - C0, want - C0)
+            print(f"  {la},{lb}->{lc} {ext} {np.dtype(dtype).name} {kern} rel={err:.2e}")
+            assert err < TOL[tolkey], (la, lb, lc, ext, kern, err)

@@ -30,28 +30,14 @@
 
 #include <type_traits>
 
-// Layout switches, measured on complex<float> 48^4 contractions in the four
-// gather-direction combinations: expanded A (+11 % on 144 x 144 tiles, a
-// loss on 64 x 64 where the larger LDS tile costs occupancy), quad staging
-// of A (+5..15 %) and of B (64 x 64 tiles only), writes after the stage's
-// MFMAs (0) rather than interleaved with them (1: at half, 2: spread), and
-// the chunk-swizzled unpadded rows (1) that helped one direction combination
-// and lost on two others.
-#ifndef T1M_BFAEXP
-#define T1M_BFAEXP 1
-#endif
-#ifndef T1M_BFQA
-#define T1M_BFQA 1
-#endif
-#ifndef T1M_BFQB
-#define T1M_BFQB 1
-#endif
-#ifndef T1M_BFMID
-#define T1M_BFMID 0
-#endif
-#ifndef T1M_BFSWZ
-#define T1M_BFSWZ 0
-#endif
+// Layout choices, measured on complex<float> 48^4 contractions in the four
+// gather-direction combinations (timing variants built from switches that
+// no longer exist): expanded A (+11 % on 144 x 144 tiles, a loss on 64 x 64
+// where the larger LDS tile costs occupancy), quad staging of A (+5..15 %)
+// and of B (64 x 64 tiles only).  Also measured and dropped: LDS writes
+// inside the MFMA stream instead of after it (no gain), and unpadded rows
+// with XOR-swizzled 16-byte chunks (+3 % on one direction combination, -5 %
+// and -10 % on two others).
 
 namespace t1m_hip {
 
@@ -151,22 +137,8 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   constexpr bool XT = NBT % WGN != 0;
   constexpr int XB = XT ? (MB + WGN - 1) / WGN : 0;
   constexpr int XB_ALLOC = XT ? XB : 1;
-  // LDS rows of BK words.  SWZ: no padding, the 16-byte chunks of row r are
-  // permuted, chunk c stored at c ^ ((r / 2) mod 4); the fragment reads
-  // (16 consecutive rows, one chunk each) and the stores of both gather
-  // directions then spread over the banks.  Otherwise rows are padded by 4
-  // words.
-  constexpr bool SWZ = T1M_BFSWZ != 0;
-  constexpr int PITCH = SWZ ? BK : BK + 4;   // words per LDS row
-  static_assert(!SWZ || BK == 16, "swizzle assumes four chunks per row");
-  auto col = [](int r, int k) { return SWZ ? ((((k >> 2) ^ ((r >> 1) & 3)) << 2) | (k & 3)) : k; };
-  static_assert(BK % 16 == 0, "16 complex k per MFMA");
-  static_assert(WTM % 8 == 0 && BN % 16 == 0, "tile shape");
-  static_assert(NBT - NBW * WGN <= 1, "at most one shared column block");
-
-  // AEXP: A's planes hold the 1M-expanded real rows 2i + s, written once per
-  // element, instead of the raw (re, im) words expanded by every reader.
-  constexpr bool AEXP = T1M_BFAEXP != 0 && BM >= 144;
+  constexpr int PITCH = BK + 4;   // words per LDS row (padded: conflict-light reads)
+  constexpr bool AEXP = BM >= 144;
   constexpr int AROWS = AEXP ? 2 * BM : BM;
   __shared__ __attribute__((aligned(16))) uint32_t sAh[2][AROWS][PITCH];
   __shared__ __attribute__((aligned(16))) uint32_t sAl[2][AROWS][PITCH];
@@ -187,7 +159,7 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   // single words, consecutive lanes filling consecutive words of a row.
   // (Measured: quads pay for A at both tile sizes and for B only on the
   // 64 x 64 tiles.)
-  constexpr bool QA = AF && T1M_BFQA != 0, QB = BF && T1M_BFQB != 0 && BM < 144;
+  constexpr bool QA = AF, QB = BF && BM < 144;
   using GA = std::conditional_t<QA, QuadGather<Idx, BM, BK, NT>,
                                 TileGather<float2, Idx, BM, BK, NT, AF>>;
   using GB = std::conditional_t<QB, QuadGather<Idx, BN, BK, NT>,
@@ -225,13 +197,13 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
             h0[t] = exp0(hi[t]); h1[t] = exp1(hi[t]);
             l0[t] = exp0(lo[t]); l1[t] = exp1(lo[t]);
           }
-          *reinterpret_cast<u32x4*>(&sAh[buf][2 * x][col(2 * x, kw)]) = h0;
-          *reinterpret_cast<u32x4*>(&sAh[buf][2 * x + 1][col(2 * x + 1, kw)]) = h1;
-          *reinterpret_cast<u32x4*>(&sAl[buf][2 * x][col(2 * x, kw)]) = l0;
-          *reinterpret_cast<u32x4*>(&sAl[buf][2 * x + 1][col(2 * x + 1, kw)]) = l1;
+          *reinterpret_cast<u32x4*>(&sAh[buf][2 * x][kw]) = h0;
+          *reinterpret_cast<u32x4*>(&sAh[buf][2 * x + 1][kw]) = h1;
+          *reinterpret_cast<u32x4*>(&sAl[buf][2 * x][kw]) = l0;
+          *reinterpret_cast<u32x4*>(&sAl[buf][2 * x + 1][kw]) = l1;
         } else {
-          *reinterpret_cast<u32x4*>(&sAh[buf][x][col(x, kw)]) = hi;
-          *reinterpret_cast<u32x4*>(&sAl[buf][x][col(x, kw)]) = lo;
+          *reinterpret_cast<u32x4*>(&sAh[buf][x][kw]) = hi;
+          *reinterpret_cast<u32x4*>(&sAl[buf][x][kw]) = lo;
         }
       }
     } else {
@@ -240,13 +212,13 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
       if (GA::valid(utid, u)) {
         const int x = GA::xc(e), k = GA::kc(e);
         if constexpr (AEXP) {
-          sAh[buf][2 * x][col(2 * x, k)] = exp0(hi);
-          sAh[buf][2 * x + 1][col(2 * x + 1, k)] = exp1(hi);
-          sAl[buf][2 * x][col(2 * x, k)] = exp0(lo);
-          sAl[buf][2 * x + 1][col(2 * x + 1, k)] = exp1(lo);
+          sAh[buf][2 * x][k] = exp0(hi);
+          sAh[buf][2 * x + 1][k] = exp1(hi);
+          sAl[buf][2 * x][k] = exp0(lo);
+          sAl[buf][2 * x + 1][k] = exp1(lo);
         } else {
-          sAh[buf][x][col(x, k)] = hi;
-          sAl[buf][x][col(x, k)] = lo;
+          sAh[buf][x][k] = hi;
+          sAl[buf][x][k] = lo;
         }
       }
     }
@@ -265,16 +237,16 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
       }
       if (GB::valid(utid, u)) {
         const int x = GB::xc(e), kw = 4 * GB::kq(e);
-        *reinterpret_cast<u32x4*>(&sBh[buf][x][col(x, kw)]) = hi;
-        *reinterpret_cast<u32x4*>(&sBl[buf][x][col(x, kw)]) = lo;
+        *reinterpret_cast<u32x4*>(&sBh[buf][x][kw]) = hi;
+        *reinterpret_cast<u32x4*>(&sBl[buf][x][kw]) = lo;
       }
     } else {
       uint32_t hi, lo;
       split_pair(gb.val[u].x, gb.val[u].y, hi, lo);
       if (GB::valid(utid, u)) {
         const int x = GB::xc(e), k = GB::kc(e);
-        sBh[buf][x][col(x, k)] = hi;
-        sBl[buf][x][col(x, k)] = lo;
+        sBh[buf][x][k] = hi;
+        sBl[buf][x][k] = lo;
       }
     }
   };
@@ -319,12 +291,8 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
     for (int t = 0; t < 4; ++t) r[t] = __builtin_amdgcn_perm(w[t], w[t], psel) ^ pxor;
     return r;
   };
-  // Word column of this lane's chunk g in the rows it reads: A rows are
-  // 2(i0 + 8mb) + l16 (expanded) or i0 + 8mb + l16/2, B rows j0 + 16nb +
-  // l16, with i0, j0 multiples of 8, so the swizzle is the same for every
-  // row block.
-  const int colA = col(AEXP ? l16 : l16 >> 1, 4 * g);
-  const int colB = col(l16, 4 * g);
+  // Word column of this lane's 4 complex k in the rows it reads.
+  const int colA = 4 * g, colB = 4 * g;
   auto ldsA = [&](const uint32_t (*plane)[PITCH], int row) {
     if constexpr (AEXP)
       return *reinterpret_cast<const u32x4*>(&plane[2 * row + (odd ? 1 : 0)][colA]);
@@ -337,20 +305,8 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
 
   // B fragments stay in registers for a k-chunk; A fragments stream per row
   // block.  Three products per tile: lo*hi and hi*lo first, then hi*hi.
-  // The next stage's LDS writes (NU units) go after the stage's MFMAs
-  // (T1M_BFMID 0), or inside the MFMA stream: all after row-block group
-  // G/2 - 1 (1), or unit i after group S + i (G - S) / NU, S = G / 2 (2).
-  constexpr int G = MB + (XT ? XB : 0);
-  constexpr int S = T1M_BFMID == 0 ? G : G / 2;
-  auto slot = [&](int i) { return T1M_BFMID == 1 ? S - 1 : S + (i * (G - S)) / NU; };
-  auto compute = [&](int buf, int wbuf, int p0, bool wr) {
+  auto compute = [&](int buf) {
     static_assert(BK == 16, "one MFMA k-chunk per stage");
-    auto after_group = [&](int m) {
-      if (!wr) return;
-#pragma unroll
-      for (int i = 0; i < NU; ++i)
-        if (slot(i) == m) write_unit(wbuf, p0, i);
-    };
     u32x4 bh[NBW], bl[NBW];
 #pragma unroll
     for (int nb = 0; nb < NBW; ++nb) {
@@ -367,7 +323,6 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
         acc[mb][nb] = mfma_bf16(bh[nb], al, acc[mb][nb]);
         acc[mb][nb] = mfma_bf16(bh[nb], ah, acc[mb][nb]);
       }
-      after_group(mb);
     }
     if constexpr (XT) {
       const u32x4 xbh = ldsB(sBh[buf], bnx);
@@ -379,10 +334,8 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
         accx[x] = mfma_bf16(xbl, ah, accx[x]);
         accx[x] = mfma_bf16(xbh, al, accx[x]);
         accx[x] = mfma_bf16(xbh, ah, accx[x]);
-        after_group(MB + x);
       }
     }
-    if (wr && T1M_BFMID == 0) swrite(wbuf, p0);
   };
 
   const int nst = (K - kbeg + BK - 1) / BK;
@@ -403,10 +356,11 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
     gb.load_data(B);
     ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
     gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
-    compute(s & 1, (s + 1) & 1, kbeg + (s + 1) * BK, true);
+    compute(s & 1);
+    swrite((s + 1) & 1, kbeg + (s + 1) * BK);
     __syncthreads();
   }
-  compute((nst - 1) & 1, 0, 0, false);
+  compute((nst - 1) & 1);
 
   float* __restrict__ Cr = static_cast<float*>(p.C);
   const bool cplx_alpha = p.alpha_im != 0.0;

@@ -41,55 +41,9 @@
 #include <cstring>
 #include <type_traits>
 
-// K depth of one LDS stage for the 144 x 144 FP32 tiles.
-#ifndef T1M_BK144
-#define T1M_BK144 16
-#endif
-// LDS layout choices per precision (measured on c64 48^4 and c128 tiles):
-// pre-expanded A pays for FP32, whose MFMAs are short enough that the
-// per-chunk 1M selects and their hazard waits showed; FP64 MFMAs take twice
-// as long and the doubled A tile costs more than the selects.
-#ifndef T1M_AEXP32
-#define T1M_AEXP32 1
-#endif
-#ifndef T1M_AEXP64
-#define T1M_AEXP64 0
-#endif
-#ifndef T1M_BSWZ32
-#define T1M_BSWZ32 1
-#endif
-#ifndef T1M_BSWZ64
-#define T1M_BSWZ64 1
-#endif
-// Mid-stage LDS writes (next stage written after half of this stage's
-// MFMAs) for tiles below 144 x 144, which have the registers to spare.
-#ifndef T1M_MIDW
-#define T1M_MIDW 1
-#endif
-// Fine-grained operand pipelining (A operands T1M_FINE_LA row groups ahead).
-#ifndef T1M_FINE
-#define T1M_FINE 1
-#endif
-#ifndef T1M_FINE_LA
-#define T1M_FINE_LA 2
-#endif
-#ifndef T1M_FINE_MIDW
-#define T1M_FINE_MIDW 1
-#endif
-// Wave layout of the FP64 64 x 64 tiles: 4 x 2 waves (16 x 32 complex per
-// wave, 2 waves per SIMD) measured 23 % faster than 2 x 2 at 1024 x 1024 x
-// 256 and 3 % at 2304^3.
-#ifndef T1M_F64_WGM
-#define T1M_F64_WGM 4
-#endif
-#ifndef T1M_F64_WGN
-#define T1M_F64_WGN 2
-#endif
-// Wave priority (s_setprio) around the MFMA stream: 0 none, 1 raised while
-// issuing each MFMA group, 2 raised for the whole stage except the writes.
-#ifndef T1M_PRIO
-#define T1M_PRIO 2
-#endif
+// Design choices below were measured on the 48^4 complex<float> and the
+// 1024 x 1024 x 256 / 2304^3 complex<double> contractions (timing variants
+// built from switches that no longer exist; results kept with each choice).
 
 namespace t1m_hip {
 
@@ -109,13 +63,13 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   constexpr bool XT = NBT % WGN != 0;    // one shared column block
   constexpr int XB = XT ? (MB + WGN - 1) / WGN : 0;
   constexpr int XB_ALLOC = XT ? XB : 1;
-  constexpr bool AEXP = sizeof(T) == 4 ? T1M_AEXP32 != 0 : T1M_AEXP64 != 0;
-  constexpr bool BSWZ = sizeof(T) == 4 ? T1M_BSWZ32 != 0 : T1M_BSWZ64 != 0;
-  constexpr bool FINE = T1M_FINE != 0;
-  // (64-bit offsets on the 144 x 144 tiles leave no registers for the
-  // mid-stage writes.)
-  constexpr bool MIDW = FINE ? (T1M_FINE_MIDW != 0 && (sizeof(Idx) == 4 || BM * BN < 144 * 144))
-                             : (T1M_MIDW != 0 && BM * BN < 144 * 144);
+  // A pre-expanded in LDS for FP32 (+7.5 %: no per-chunk selects and their
+  // MFMA hazard waits); FP64 MFMAs take twice as long and the doubled A tile
+  // cost more than the selects (-9 %).
+  constexpr bool AEXP = sizeof(T) == 4;
+  // Next stage's LDS writes in the middle of the stage; 64-bit offsets on
+  // the 144 x 144 tiles leave no registers for them.
+  constexpr bool MIDW = sizeof(Idx) == 4 || BM * BN < 144 * 144;
   // LDS layouts.  A holds each complex element twice, pre-expanded for the
   // two row parities of 1M: (re, -im) for even lanes, (im, re) for odd ones,
   // so the MFMA operands come straight from the LDS reads.  Both tiles are
@@ -130,17 +84,17 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   using AU = std::conditional_t<AEXP, AExp, C2>;
   constexpr int SA = 256 / int(sizeof(AU));   // units per 256-byte bank sweep
   constexpr int SB = 256 / int(sizeof(C2));
-  constexpr int PADA = AEXP ? ((SA / 2 - BM % SA) % SA + SA) % SA : (sizeof(T) == 4 ? 8 : 4);
-  constexpr int PADB = BSWZ ? (((SB / 2 - BN % SB) + SB) % SB) : (sizeof(T) == 4 ? 8 : 4);
+  constexpr int PADA = AEXP ? ((SA / 2 - BM % SA) % SA + SA) % SA : 4;
+  constexpr int PADB = ((SB / 2 - BN % SB) + SB) % SB;
   constexpr int LDA = BM + PADA, LDB = BN + PADB;
   static_assert(!AEXP || (LDA * int(sizeof(AU))) % 256 == 128, "A pitch");
-  static_assert(!BSWZ || (LDB * int(sizeof(C2))) % 256 == 128, "B pitch");
+  static_assert((LDB * int(sizeof(C2))) % 256 == 128, "B pitch");
   static_assert(WTM % 8 == 0 && BN % 16 == 0, "tile shape");
   static_assert(NBT - NBW * WGN <= 1, "at most one shared column block");
   static_assert(BK % 4 == 0, "k chunk");
   // A read group spans 8 aligned units, a B group 16.
   auto swa = [](int k) { return AEXP ? (k >> 1) & (SA / 2 - 1 < 7 ? SA / 2 - 1 : 7) : 0; };
-  auto swb = [](int k) { return BSWZ ? (k >> 1) & (SB / 2 - 1 < 15 ? SB / 2 - 1 : 15) : 0; };
+  auto swb = [](int k) { return (k >> 1) & (SB / 2 - 1 < 15 ? SB / 2 - 1 : 15); };
 
   __shared__ AU sA[2][BK][LDA];
   __shared__ C2 sB[2][BK][LDB];
@@ -215,14 +169,8 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
     xmb[x] = mb < MB ? mb : MB - 1;
   }
 
-  // Fragments of one 4-k chunk: A as the two 1M operands of this lane's
-  // row parity (AEXP) or as the raw (re, im) pair, B as (re, im).
-  struct Frag {
-    C2 a[MB];
-    C2 b[NBW];
-    C2 ax[XB_ALLOC];
-    C2 bx;
-  };
+  // A operands: the two 1M operands of this lane's row parity (AEXP) or
+  // the raw (re, im) pair.
   auto aread = [&](int buf, int q, int row0) -> C2 {
     const int x = row0 + (((l16 >> 1) ^ swa(q)) - (l16 >> 1));
     if constexpr (AEXP) {
@@ -230,19 +178,6 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
       return u[odd ? 1 : 0];
     } else {
       return sA[buf][q][x];
-    }
-  };
-  auto load_frag = [&](int buf, int kc, Frag& f) {
-    const int q = kc * 4 + g;
-    const int bsw = (l16 ^ swb(q)) - l16;
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) f.a[mb] = aread(buf, q, am0 + mb * 8);
-#pragma unroll
-    for (int nb = 0; nb < NBW; ++nb) f.b[nb] = sB[buf][q][bn0 + bsw + nb * 16];
-    if constexpr (XT) {
-      f.bx = sB[buf][q][bnx + bsw];
-#pragma unroll
-      for (int x = 0; x < XB; ++x) f.ax[x] = aread(buf, q, am0 + xmb[x] * 8);
     }
   };
   // Two MFMAs per tile: B's real parts against A_r's even real columns,
@@ -256,41 +191,18 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
       im = odd ? a.x : -a.y;   // A_r[2i+s][2q+1]
     }
   };
-  auto mma = [&](const Frag& f) {
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) {
-      T re, im;
-      operands(f.a[mb], re, im);
-#pragma unroll
-      for (int nb = 0; nb < NBW; ++nb) {
-        acc[mb][nb] = mfma16(f.b[nb].x, re, acc[mb][nb]);
-        acc[mb][nb] = mfma16(f.b[nb].y, im, acc[mb][nb]);
-      }
-    }
-    if constexpr (XT) {
-#pragma unroll
-      for (int x = 0; x < XB; ++x) {
-        T re, im;
-        operands(f.ax[x], re, im);
-        accx[x] = mfma16(f.bx.x, re, accx[x]);
-        accx[x] = mfma16(f.bx.y, im, accx[x]);
-      }
-    }
-  };
-  // Chunk kc + 1's LDS reads are issued before chunk kc's MFMAs (the
-  // scheduling barriers keep the compiler from sinking them), so a wave
-  // never waits on LDS latency inside a stage.
-  // `mid` runs after the first half of the stage's MFMAs are issued: the
-  // main loop puts the next stage's LDS writes there, so they overlap the
-  // matrix pipe instead of idling it before the barrier.
-  // Fine-grained form: B fragments one chunk ahead, A operands LA row
+  // Operand pipeline: B fragments one 4-k chunk ahead, A operands LA row
   // groups ahead of the MFMAs that use them (a rolling window over the
-  // KCH x NG (chunk, row group) steps of the stage), which needs far fewer
-  // registers than a whole chunk of fragments in flight.
+  // KCH x NG (chunk, row group) steps of a stage; the scheduling barriers
+  // keep the compiler from sinking the LDS reads).  Loading whole chunks of
+  // fragments ahead needed far more registers for the same speed; LA from 1
+  // to 6 measured alike.  `mid` (the next stage's LDS writes) runs after
+  // half the stage's MFMAs.  The wave's priority is raised over the MFMA
+  // stream and lowered for the writes (+1-2 %).
   constexpr int KCH = BK / 4;
   constexpr int NG = MB + (XT ? XB : 0);     // row groups per chunk
   constexpr int TOT = KCH * NG;
-  constexpr int LA = T1M_FINE_LA;
+  constexpr int LA = 2;
   struct BFr {
     C2 b[NBW];
     C2 bx;
@@ -325,7 +237,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   auto compute_fine = [&](int buf, auto&& mid) {
     BFr fb[2];
     C2 aw[LA + 1];
-    if (T1M_PRIO == 2) __builtin_amdgcn_s_setprio(1);
+    __builtin_amdgcn_s_setprio(1);
     load_b(buf, 0, fb[0]);
 #pragma unroll
     for (int t = 0; t < LA; ++t) aw[t] = load_a(buf, t);
@@ -335,39 +247,19 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
       if (grp == 0 && kc + 1 < KCH) load_b(buf, kc + 1, fb[(kc + 1) & 1]);
       if (t + LA < TOT) aw[(t + LA) % (LA + 1)] = load_a(buf, t + LA);
       __builtin_amdgcn_sched_barrier(0);
-      if (T1M_PRIO == 1) __builtin_amdgcn_s_setprio(1);
       mma_grp(grp, aw[t % (LA + 1)], fb[kc & 1]);
-      if (T1M_PRIO == 1) __builtin_amdgcn_s_setprio(0);
       __builtin_amdgcn_sched_barrier(0);
       if (MIDW && t == TOT / 2 - 1) {
-        if (T1M_PRIO == 2) __builtin_amdgcn_s_setprio(0);
+        __builtin_amdgcn_s_setprio(0);
         mid();
-        if (T1M_PRIO == 2) __builtin_amdgcn_s_setprio(1);
+        __builtin_amdgcn_s_setprio(1);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    if (T1M_PRIO == 2) __builtin_amdgcn_s_setprio(0);
+    __builtin_amdgcn_s_setprio(0);
     if (!MIDW) mid();
   };
-  auto compute = [&](int buf, auto&& mid) {
-    if constexpr (FINE) {
-      compute_fine(buf, mid);
-    } else {
-      Frag f[2];
-      load_frag(buf, 0, f[0]);
-#pragma unroll
-      for (int kc = 0; kc < KCH; ++kc) {
-        if (kc + 1 < KCH) load_frag(buf, kc + 1, f[(kc + 1) & 1]);
-        __builtin_amdgcn_sched_barrier(0);
-        mma(f[kc & 1]);
-        __builtin_amdgcn_sched_barrier(0);
-        if (kc == (MIDW ? KCH / 2 - 1 : KCH - 1)) {
-          mid();
-          __builtin_amdgcn_sched_barrier(0);
-        }
-      }
-    }
-  };
+  auto compute = [&](int buf, auto&& mid) { compute_fine(buf, mid); };
 
   // ---- main loop: double-buffered LDS, one barrier per stage ------------
   // Iteration s multiplies stage s from LDS while the data loads of stage
@@ -533,9 +425,11 @@ hipError_t launch_1m(const GemmProblem& g, hipStream_t s) {
 template <typename Idx>
 hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
   switch (kernel) {
-    case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, T1M_BK144, 2, 4>(g, s);
+    case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, 16, 2, 4>(g, s);
     case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 16, 2, 2>(g, s);
-    case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16, T1M_F64_WGM, T1M_F64_WGN>(g, s);
+    // 4 x 2 waves (16 x 32 complex per wave, 2 waves per SIMD): +23 % over
+    // 2 x 2 at 1024 x 1024 x 256, +3 % at 2304^3.
+    case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16, 4, 2>(g, s);
     case K_C128_F64_32: return launch_1m<double, Idx, 32, 32, 16, 2, 2>(g, s);
     default: return launch_simple<Idx>(g, s);
   }

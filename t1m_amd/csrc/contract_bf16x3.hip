@@ -362,37 +362,52 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   }
   compute((nst - 1) & 1);
 
+  // Epilogue (as in the FP32 1M kernel): complex rows i0 + am0 + 8 r' (re
+  // or im by lane parity) times 4 columns per 16-column block.
   float* __restrict__ Cr = static_cast<float*>(p.C);
   const bool cplx_alpha = p.alpha_im != 0.0;
   const float ar = float(p.alpha_re), ai = float(p.alpha_im);
-  auto store = [&](const f32x4& v, int64_t i, int64_t jb) {
-    const bool iok = i < M;
-    const Idx rc = iok ? p.rsC[i] : Idx(0);
+  constexpr int NR = MB + (XT ? XB : 0);
+  constexpr int NC = NBW + (XT ? 1 : 0);
+  Idx rofs[NR];
+  bool rok[NR];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      float val = v[r];
-      if (cplx_alpha) {
-        const float other = __shfl_xor(val, 1);
-        val = odd ? ar * val + ai * other : ar * val - ai * other;
-      } else {
-        val = ar * val;
-      }
-      const int64_t j = jb + 4 * g + r;
-      if (iok && j < N)
-        accumulate(Cr + 2 * int64_t(rc + p.csC[j]) + (odd ? 1 : 0), val, p.ksplit > 1);
+  for (int r = 0; r < NR; ++r) {
+    const int blk = r < MB ? r : wn * XB + (r - MB);
+    const int64_t i = i0 + am0 + (blk < MB ? blk : MB - 1) * 8;
+    rok[r] = i < M && blk < MB;
+    rofs[r] = rok[r] ? p.rsC[i] : Idx(0);
+  }
+  Idx cofs[NC][4];
+  bool cok[NC][4];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t jb = c < NBW ? j0 + wn * NBW * 16 + c * 16 : j0 + NBW * WGN * 16;
+      const int64_t j = jb + 4 * g + q;
+      cok[c][q] = j < N;
+      cofs[c][q] = cok[c][q] ? p.csC[j] : Idx(0);
     }
+  auto scaled = [&](float v) {
+    if (cplx_alpha) {
+      const float other = swap_pair(v);
+      return odd ? ar * v + ai * other : ar * v - ai * other;
+    }
+    return ar * v;
   };
-#pragma unroll
-  for (int mb = 0; mb < MB; ++mb) {
-    const int64_t i = i0 + am0 + mb * 8;
-#pragma unroll
-    for (int nb = 0; nb < NBW; ++nb) store(acc[mb][nb], i, j0 + wn * NBW * 16 + nb * 16);
-  }
-  if constexpr (XT) {
-#pragma unroll
-    for (int x = 0; x < XB; ++x)
-      if (wn * XB + x < MB) store(accx[x], i0 + am0 + (wn * XB + x) * 8, j0 + NBW * WGN * 16);
-  }
+  auto value = [&](int r, int c, int q) -> float {
+    if constexpr (XT) {
+      if (r >= MB) return scaled(accx[r - MB][q]);
+    }
+    return scaled(acc[r][c][q]);
+  };
+  auto has = [](int r, int c) { return r < MB ? c < NBW : c == NC - 1; };
+  auto ok = [&](int r, int c, int q) { return rok[r] && cok[c][q]; };
+  auto addr = [&](int r, int c, int q) {
+    return Cr + 2 * int64_t(rofs[r] + cofs[c][q]) + (odd ? 1 : 0);
+  };
+  store_tile<float, NR, NC, 3>(p.ksplit > 1, has, ok, value, addr);
 }
 
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>

@@ -288,41 +288,58 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   compute((nst - 1) & 1, [] {});
 
   // ---- epilogue: C[rsC[i] + csC[j]] += alpha * acc -----------------------
+  // This lane's outputs: complex rows i (one per row block; re or im by
+  // lane parity) times columns j (4 per 16-column block).  All row and
+  // column offsets are loaded first (those of missing outputs are 0, a
+  // valid element), then store_tile updates C.
   T* __restrict__ Cr = static_cast<T*>(p.C);
   const bool cplx_alpha = p.alpha_im != 0.0;
   const T ar = T(p.alpha_re), ai = T(p.alpha_im);
-  auto store = [&](const Acc& v, int64_t i, int64_t jb) {
-    const bool iok = i < M;
-    const Idx rc = iok ? p.rsC[i] : Idx(0);
+  constexpr int NR = MB + (XT ? XB : 0);        // row blocks (shared ones last)
+  constexpr int NC = NBW + (XT ? 1 : 0);        // column blocks (shared one last)
+  Idx rofs[NR];
+  bool rok[NR];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      T val = v[r];
-      if (cplx_alpha) {
-        const T other = __shfl_xor(val, 1);
-        val = odd ? ar * val + ai * other : ar * val - ai * other;
-      } else {
-        val = ar * val;
-      }
-      // MFMA output row of (lane, r): 4(lane/16) + r for the f32 16x16x4
-      // form, lane/16 + 4r for the f64 one.
-      const int64_t j = jb + (sizeof(T) == 4 ? 4 * g + r : g + 4 * r);
-      if (iok && j < N) {
-        accumulate(Cr + 2 * int64_t(rc + p.csC[j]) + (odd ? 1 : 0), val, p.ksplit > 1);
-      }
+  for (int r = 0; r < NR; ++r) {
+    const int blk = r < MB ? r : wn * XB + (r - MB);
+    const int64_t i = i0 + am0 + (r < MB ? r : (blk < MB ? blk : MB - 1)) * 8;
+    rok[r] = i < M && (r < MB || blk < MB);
+    rofs[r] = rok[r] ? p.rsC[i] : Idx(0);
+  }
+  Idx cofs[NC][4];
+  bool cok[NC][4];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // MFMA output row of (lane, q): 4(lane/16) + q for the f32 16x16x4
+      // form, lane/16 + 4q for the f64 one.
+      const int64_t jb = c < NBW ? j0 + wn * NBW * 16 + c * 16 : j0 + NBW * WGN * 16;
+      const int64_t j = jb + (sizeof(T) == 4 ? 4 * g + q : g + 4 * q);
+      cok[c][q] = j < N;
+      cofs[c][q] = cok[c][q] ? p.csC[j] : Idx(0);
     }
+  auto scaled = [&](T v) {
+    if (cplx_alpha) {
+      const T other = swap_pair(v);     // the neighbour lane holds the other part
+      return odd ? ar * v + ai * other : ar * v - ai * other;
+    }
+    return ar * v;
   };
-#pragma unroll
-  for (int mb = 0; mb < MB; ++mb) {
-    const int64_t i = i0 + am0 + mb * 8;
-#pragma unroll
-    for (int nb = 0; nb < NBW; ++nb)
-      store(acc[mb][nb], i, j0 + wn * NBW * 16 + nb * 16);
-  }
-  if constexpr (XT) {
-#pragma unroll
-    for (int x = 0; x < XB; ++x)
-      if (wn * XB + x < MB) store(accx[x], i0 + am0 + (wn * XB + x) * 8, j0 + NBW * WGN * 16);
-  }
+  auto value = [&](int r, int c, int q) -> T {
+    if constexpr (XT) {
+      if (r >= MB) return scaled(accx[r - MB][q]);
+    }
+    return scaled(acc[r][c][q]);
+  };
+  // (row block, column block) pairs this lane holds: full blocks x full
+  // columns, and the shared row blocks x the shared column.
+  auto has = [](int r, int c) { return r < MB ? c < NBW : c == NC - 1; };
+  auto ok = [&](int r, int c, int q) { return rok[r] && cok[c][q]; };
+  auto addr = [&](int r, int c, int q) {
+    return Cr + 2 * int64_t(rofs[r] + cofs[c][q]) + (odd ? 1 : 0);
+  };
+  store_tile<T, NR, NC, 3>(p.ksplit > 1, has, ok, value, addr);
 }
 
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,

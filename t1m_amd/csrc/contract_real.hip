@@ -154,28 +154,43 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
   }
   compute((nst - 1) & 1);
 
+  // Epilogue: this lane holds rows i0 + am0 + 16 r' (one per row block)
+  // times 4 columns per 16-column block; offsets first, then store_tile.
   T* __restrict__ C = static_cast<T*>(p.C);
   const T alpha = T(p.alpha_re);
-  auto store = [&](const Acc& v, int64_t i, int64_t jb) {
-    const bool iok = i < M;
-    const Idx rc = iok ? p.rsC[i] : Idx(0);
+  constexpr int NR = MB + (XT ? XB : 0);        // row blocks (shared ones last)
+  constexpr int NC = NBW + (XT ? 1 : 0);        // column blocks (shared one last)
+  Idx rofs[NR];
+  bool rok[NR];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      // MFMA output row of (lane, r): 4(lane/16) + r (f32), lane/16 + 4r (f64).
-      const int64_t j = jb + (sizeof(T) == 4 ? 4 * g + r : g + 4 * r);
-      if (iok && j < N) accumulate(C + int64_t(rc + p.csC[j]), alpha * v[r], p.ksplit > 1);
-    }
-  };
-#pragma unroll
-  for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-    for (int nb = 0; nb < NBW; ++nb)
-      store(acc[mb][nb], i0 + am0 + mb * 16, j0 + wn * NBW * 16 + nb * 16);
-  if constexpr (XT) {
-#pragma unroll
-    for (int x = 0; x < XB; ++x)
-      if (wn * XB + x < MB) store(accx[x], i0 + am0 + (wn * XB + x) * 16, j0 + NBW * WGN * 16);
+  for (int r = 0; r < NR; ++r) {
+    const int blk = r < MB ? r : wn * XB + (r - MB);
+    const int64_t i = i0 + am0 + (blk < MB ? blk : MB - 1) * 16;
+    rok[r] = i < M && blk < MB;
+    rofs[r] = rok[r] ? p.rsC[i] : Idx(0);
   }
+  Idx cofs[NC][4];
+  bool cok[NC][4];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // MFMA output row of (lane, q): 4(lane/16) + q (f32), lane/16 + 4q (f64).
+      const int64_t jb = c < NBW ? j0 + wn * NBW * 16 + c * 16 : j0 + NBW * WGN * 16;
+      const int64_t j = jb + (sizeof(T) == 4 ? 4 * g + q : g + 4 * q);
+      cok[c][q] = j < N;
+      cofs[c][q] = cok[c][q] ? p.csC[j] : Idx(0);
+    }
+  auto value = [&](int r, int c, int q) -> T {
+    if constexpr (XT) {
+      if (r >= MB) return alpha * accx[r - MB][q];
+    }
+    return alpha * acc[r][c][q];
+  };
+  auto has = [](int r, int c) { return r < MB ? c < NBW : c == NC - 1; };
+  auto ok = [&](int r, int c, int q) { return rok[r] && cok[c][q]; };
+  auto addr = [&](int r, int c, int q) { return C + int64_t(rofs[r] + cofs[c][q]); };
+  store_tile<T, NR, NC, 3>(p.ksplit > 1, has, ok, value, addr);
 }
 
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>

@@ -53,6 +53,68 @@ __device__ inline void batch_shift(KArgs<Idx>& q, int z) {
   }
 }
 
+// The value of the neighbouring lane (lanes 2t and 2t+1 swap), by a DPP
+// quad permutation [1, 0, 3, 2]: no LDS round trip.
+__device__ inline float swap_pair(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1,
+                                                            0xF, 0xF, false));
+}
+__device__ inline double swap_pair(double v) {
+  const int2 w = __builtin_bit_cast(int2, v);
+  int2 r;
+  r.x = __builtin_amdgcn_mov_dpp(w.x, 0xB1, 0xF, 0xF, false);
+  r.y = __builtin_amdgcn_mov_dpp(w.y, 0xB1, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, r);
+}
+
+// Epilogue of a tile: C[addr(r, c, q)] += val(r, c, q) for this lane's
+// outputs, NR row groups x NC column groups x 4, where has(r, c) says which
+// (row group, column group) pairs the lane holds (compile-time) and ok(r, c,
+// q) which outputs exist (addr of the others must still be a valid address
+// of C).  With `atomic` (K split over workgroups) the adds are atomic.
+// Otherwise C is read for GR row groups at a time, then updated and written:
+// a compiler cannot move a C read above an earlier C write (they might
+// alias), so an element-by-element read-modify-write would wait out a
+// memory latency per element.
+template <typename T, int NR, int NC, int GR, typename Has, typename Ok, typename Val,
+          typename Addr>
+__device__ __forceinline__ void store_tile(bool atomic, Has has, Ok ok, Val val, Addr addr) {
+  if (atomic) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (has(r, c))
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const T v = val(r, c, q);
+            if (ok(r, c, q)) unsafeAtomicAdd(addr(r, c, q), v);
+          }
+    return;
+  }
+#pragma unroll
+  for (int r0 = 0; r0 < NR; r0 += GR) {
+    T cv[GR][NC][4];
+#pragma unroll
+    for (int r = r0; r < r0 + GR && r < NR; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (has(r, c))
+#pragma unroll
+          for (int q = 0; q < 4; ++q) cv[r - r0][c][q] = *addr(r, c, q);
+#pragma unroll
+    for (int r = r0; r < r0 + GR && r < NR; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (has(r, c))
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const T v = val(r, c, q);
+            if (ok(r, c, q)) *addr(r, c, q) = cv[r - r0][c][q] + v;
+          }
+  }
+}
+
 // C[idx] += v; atomically when several workgroups share the tile (split K).
 template <typename T>
 __device__ inline void accumulate(T* dst, T v, bool atomic) {

@@ -137,6 +137,7 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
   bt->synchronize = o.synchronize;
   const hipStream_t stream = static_cast<hipStream_t>(o.stream);
   int rc;
+  std::vector<std::pair<uintptr_t, uintptr_t>> spans;   // C byte range of each instance
   for (int64_t i = 0; i < count; ++i) {
     Problem P;
     if ((rc = parse_problem(a + i, la, b + i, lb, c + i, lc, P))) return rc;
@@ -159,8 +160,12 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
     g.conj_a = cplx && o.conj_a; g.conj_b = cplx && o.conj_b;
     g.a_mfast = plan->a_mfast; g.b_kfast = plan->b_kfast;
     g.idx32 = plan->idx32; g.dtype = P.dtype;
-    g.split_k = 1;   // instances run side by side; no atomics needed
+    g.split_k = 1;   // whole tiles per workgroup (no split K)
     g.cus = 0;
+    g.c_overlap = plan->c_overlap ? 1 : 0;
+    const uintptr_t c0 = reinterpret_cast<uintptr_t>(c[i].data);
+    const int64_t es = dtype_size(P.dtype);
+    spans.emplace_back(c0 + plan->c_lo * es, c0 + (plan->c_hi + 1) * es);
     // An instance with batch labels contributes one problem per batch index.
     for (int64_t z = 0; z < plan->L; ++z) {
       GemmProblem gz = g;
@@ -173,6 +178,14 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
       bt->problems.push_back(gz);
     }
   }
+  // Instances run side by side: when two write into one range of memory,
+  // every instance adds atomically.
+  std::sort(spans.begin(), spans.end());
+  for (size_t i = 1; i < spans.size(); ++i)
+    if (spans[i].first < spans[i - 1].second) {
+      for (GemmProblem& g : bt->problems) g.c_overlap = 1;
+      break;
+    }
   if ((rc = build_persistent(*bt, o.precision))) return rc;
   if (!bt->persistent && o.autotune) {
     for (size_t i = 0; i < bt->plans.size(); ++i)

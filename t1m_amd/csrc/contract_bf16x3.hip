@@ -407,7 +407,7 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   auto addr = [&](int r, int c, int q) {
     return Cr + 2 * int64_t(rofs[r] + cofs[c][q]) + (odd ? 1 : 0);
   };
-  store_tile<float, NR, NC, 3>(p.ksplit > 1, has, ok, value, addr);
+  store_tile<float, NR, NC, 3>(p.atomic_c != 0, has, ok, value, addr);
 }
 
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>

@@ -339,7 +339,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   auto addr = [&](int r, int c, int q) {
     return Cr + 2 * int64_t(rofs[r] + cofs[c][q]) + (odd ? 1 : 0);
   };
-  store_tile<T, NR, NC, 3>(p.ksplit > 1, has, ok, value, addr);
+  store_tile<T, NR, NC, 3>(p.atomic_c != 0, has, ok, value, addr);
 }
 
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
@@ -397,13 +397,13 @@ __global__ void __launch_bounds__(256) k_contract_simple(KArgs<Idx> p) {
     }
     const T ar = T(p.alpha_re), ai = T(p.alpha_im);
     const int64_t oc = 2 * int64_t(p.rsC[i] + p.csC[j]);
-    C[oc] += ar * re - ai * im;
-    C[oc + 1] += ar * im + ai * re;
+    accumulate(C + oc, ar * re - ai * im, p.atomic_c != 0);
+    accumulate(C + oc + 1, ar * im + ai * re, p.atomic_c != 0);
   } else {
     T acc = 0;
     for (int64_t q = 0; q < p.K; ++q)
       acc = fma(A[ra + p.csA[q]], B[p.rsB[q] + cb], acc);
-    C[p.rsC[i] + p.csC[j]] += T(p.alpha_re) * acc;
+    accumulate(C + int64_t(p.rsC[i] + p.csC[j]), T(p.alpha_re) * acc, p.atomic_c != 0);
   }
 }
 

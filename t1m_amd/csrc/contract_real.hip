@@ -190,7 +190,7 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
   auto has = [](int r, int c) { return r < MB ? c < NBW : c == NC - 1; };
   auto ok = [&](int r, int c, int q) { return rok[r] && cok[c][q]; };
   auto addr = [&](int r, int c, int q) { return C + int64_t(rofs[r] + cofs[c][q]); };
-  store_tile<T, NR, NC, 3>(p.ksplit > 1, has, ok, value, addr);
+  store_tile<T, NR, NC, 3>(p.atomic_c != 0, has, ok, value, addr);
 }
 
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>

@@ -62,6 +62,7 @@ struct GemmProblem {
   int32_t dtype;     // t1m_dtype
   int32_t split_k;   // 0: split K when the tiles cannot fill the CUs; 1: never
   int32_t cus;       // compute units of the device (split-K sizing)
+  int32_t c_overlap = 0;   // distinct outputs may share an address: add atomically
   // Batch modes (labels in A, B and C): L independent problems, problem z
   // reading A, B, C at byte offsets boffA[z], boffB[z], boffC[z] (device).
   int64_t L = 1;

@@ -37,6 +37,7 @@ struct KArgs {
   int32_t tiles_m, tiles_n;
   int32_t ksplit;    // workgroups per tile along K (1: no split)
   int32_t kchunk;    // contracted extent per split, a multiple of the stage depth
+  int32_t atomic_c;  // add into C atomically (split K, or outputs that alias)
   int32_t nbatch;    // batch problems in this launch (1: none)
   const int64_t* boffA;   // byte offsets of the batch problems (nbatch > 1)
   const int64_t* boffB;
@@ -289,6 +290,7 @@ KArgs<Idx> make_args(const GemmProblem& g, int bm, int bn, int bk = 16) {
       a.ksplit = int((g.K + chunk - 1) / chunk);
     }
   }
+  a.atomic_c = (a.ksplit > 1 || g.c_overlap) ? 1 : 0;
   return a;
 }
 

@@ -52,6 +52,7 @@ struct Plan {
   std::atomic<bool> tuned{false};
   std::mutex tune_mu;
   int64_t c_lo = 0, c_hi = 0;                    // C's element span
+  bool c_overlap = false;                        // two index tuples of C may share an address
   int64_t M = 0, N = 0, K = 0, L = 1;
   int a_mfast = 1, b_kfast = 1, idx32 = 1;
   void* maps = nullptr;
@@ -84,6 +85,7 @@ int parse_problem(const t1m_tensor* a, const char* la, const t1m_tensor* b,
                   const char* lb, const t1m_tensor* c, const char* lc,
                   Problem& P);
 int choose_kernel(const Problem& P, int precision);
+bool may_alias(const std::vector<int64_t>& dims, const std::vector<int64_t>& strides);
 int get_plan(const Problem& P, int precision, hipStream_t stream,
              std::shared_ptr<Plan>& out);
 int64_t plan_cache_size();

@@ -470,6 +470,24 @@ void fill_map(MapDesc& d, const std::vector<Mode>& modes, int which, bool out32,
 
 }  // namespace
 
+// True unless the layout provably gives every index tuple its own element:
+// with the modes of extent > 1 sorted by |stride|, each stride must pass the
+// reach of all smaller ones (a zero stride, or strides that interleave,
+// fail).  Outputs that may alias are then added atomically, so every
+// contribution lands as in a serial loop.
+bool may_alias(const std::vector<int64_t>& dims, const std::vector<int64_t>& strides) {
+  std::vector<std::pair<int64_t, int64_t>> m;   // (|stride|, extent)
+  for (size_t i = 0; i < dims.size(); ++i)
+    if (dims[i] > 1) m.emplace_back(strides[i] < 0 ? -strides[i] : strides[i], dims[i]);
+  std::sort(m.begin(), m.end());
+  int64_t reach = 0;                              // largest offset of the modes so far
+  for (const auto& [s, e] : m) {
+    if (s <= reach) return true;
+    reach += (e - 1) * s;
+  }
+  return false;
+}
+
 int get_plan(const Problem& P0, int precision, hipStream_t stream,
              std::shared_ptr<Plan>& out) {
   int device = 0;
@@ -496,6 +514,7 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
     const int64_t span = (P.dc[i] > 0 ? P.dc[i] - 1 : 0) * P.sc[i];
     if (span < 0) pl.c_lo += span; else pl.c_hi += span;
   }
+  pl.c_overlap = may_alias(P.dc, P.sc);
 
   // Staging directions and traversal order of each bundle.
   pl.a_mfast = min_stride(P.m, 0) <= min_stride(P.k, 0) ? 1 : 0;
@@ -795,6 +814,7 @@ int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
   g.dtype = P.dtype;
   g.split_k = opt.deterministic ? 1 : 0;
   g.cus = cu_count();
+  g.c_overlap = plan->c_overlap ? 1 : 0;
   g.L = plan->L;
   if (plan->L > 1) {
     g.boffA = plan->boff;

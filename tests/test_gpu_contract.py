@@ -662,6 +662,50 @@ def test_batch_matches_single_calls():
     batch.close()
 
 
+def test_aliased_outputs_add_every_contribution():
+    """C views whose index tuples share addresses (a zero stride, interleaved
+    strides) get every contribution, as a serial loop over the outputs would
+    give; expected values from numpy's unbuffered np.add.at."""
+    rng = np.random.default_rng(21)
+    M, K, N = 150, 37, 70
+    for dtype in (np.complex64, np.complex128, np.float32, np.float64):
+        A = rand(rng, (M, K), dtype)
+        B = rand(rng, (K, N), dtype)
+        P = A.astype(np.complex128 if np.dtype(dtype).kind == "c" else np.float64) @ B
+        # (stride_m, stride_n, buffer length): zero stride on m, then m and n interleaved
+        for sm, sn in ((0, 1), (1, 2), (3, 1)):
+            length = (M - 1) * sm + (N - 1) * sn + 1
+            idx = (np.arange(M)[:, None] * sm + np.arange(N)[None, :] * sn).ravel()
+            buf0 = rand(rng, (length,), dtype)
+            want = buf0.astype(P.dtype)
+            np.add.at(want, idx, (0.5 * P).ravel())
+            for kern, prec, tolkey in ALL_KERNELS[dtype]:
+                buf = to_dev(buf0)
+                c = buf.as_strided((M, N), (sm, sn))
+                with forced(kern):
+                    t1m_amd.contract(to_dev(A), "ak", to_dev(B), "kn", c, "an",
+                                     precision=prec, alpha=0.5)
+                err = rel_err(to_host(buf) - buf0, want - buf0)
+                print(f"  strides ({sm},{sn}) {np.dtype(dtype).name} {kern} rel={err:.2e}")
+                assert err < TOL[tolkey], (sm, sn, dtype, kern, err)
+
+
+def test_batch_instances_sharing_one_output():
+    """Batch instances that write into one C add both contributions."""
+    rng = np.random.default_rng(22)
+    la, lb, lc = "asb", "btc", "astc"
+    A1, A2 = (rand(rng, (48, 4, 32), np.complex64) for _ in range(2))
+    B1, B2 = (rand(rng, (32, 4, 40), np.complex64) for _ in range(2))
+    C0 = rand(rng, (48, 4, 4, 40), np.complex64)
+    want = oracle_einsum(A2, la, B2, lb, oracle_einsum(A1, la, B1, lb, C0, lc), lc) - C0
+    c = to_dev(C0)
+    batch = t1m_amd.Batch([to_dev(A1), to_dev(A2)], la, [to_dev(B1), to_dev(B2)], lb, [c, c], lc)
+    batch.launch()
+    torch.cuda.synchronize()
+    assert rel_err(to_host(c) - C0, want) < TOL["fp32"]
+    batch.close()
+
+
 # ------------------------------------------------------- C++ drop-in ----
 
 def test_cpp_dropin_program():

@@ -212,6 +212,114 @@ def library_gemm(cfg, torch, a, b, c, steps):
                     "permute + matmul + permute-add into C (ttgt); comparison only"}
 
 
+def hiptensor_contract(cfg, torch, t1m, a, b, c_like, steps):
+    """Ceiling comparison, never in the product path: AMD's tensor-contraction
+    library (libhiptensor from ROCm) on the same strided device operands,
+    C += A.B (alpha = beta = 1, D = C), default algorithm, timed with HIP
+    events over `steps` calls.  Its result on a zeroed C is compared with
+    this library's, so the two compute the same contraction."""
+    import ctypes
+    try:
+        lib = ctypes.CDLL("libhiptensor.so")
+    except OSError:
+        try:
+            lib = ctypes.CDLL("/opt/rocm/lib/libhiptensor.so")
+        except OSError as e:
+            return {"status": f"unavailable ({e})"}
+    vp, i32, u32, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64
+    P = ctypes.POINTER
+    lib.hiptensorCreate.argtypes = [P(vp)]
+    lib.hiptensorCreateTensorDescriptor.argtypes = [vp, P(vp), u32, P(ctypes.c_int64),
+                                                    P(ctypes.c_int64), i32, u32]
+    lib.hiptensorCreateContraction.argtypes = [vp, P(vp), vp, P(i32), i32, vp, P(i32), i32,
+                                               vp, P(i32), i32, vp, P(i32), u32]
+    lib.hiptensorCreatePlanPreference.argtypes = [vp, P(vp), i32, i32]
+    lib.hiptensorEstimateWorkspaceSize.argtypes = [vp, vp, vp, i32, P(u64)]
+    lib.hiptensorCreatePlan.argtypes = [vp, P(vp), vp, vp, u64]
+    lib.hiptensorContract.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, u64, vp]
+    for f in ("hiptensorDestroyPlan", "hiptensorDestroyPlanPreference",
+              "hiptensorDestroyOperationDescriptor", "hiptensorDestroyTensorDescriptor",
+              "hiptensorDestroy"):
+        getattr(lib, f).argtypes = [vp]
+    cplx64 = cfg["dtype"] == "complex64"
+    dtype_code = 4 if cplx64 else 5                       # HIPTENSOR_C_32F / C_64F
+    computes = [1 << 11, 1 << 2] if cplx64 else [1 << 12, 1 << 4]   # C32F, 32F / C64F, 64F
+    made = []
+
+    def ck(st, what):
+        if st != 0:
+            raise RuntimeError(f"{what}: hiptensor status {st}")
+
+    handle = vp()
+    ck(lib.hiptensorCreate(ctypes.byref(handle)), "create")
+    try:
+        def desc(t):
+            n = t.dim()
+            lens = (ctypes.c_int64 * max(n, 1))(*t.shape)
+            strides = (ctypes.c_int64 * max(n, 1))(*t.stride())
+            d = vp()
+            ck(lib.hiptensorCreateTensorDescriptor(handle, ctypes.byref(d), n, lens, strides,
+                                                   dtype_code, 16), "tensor descriptor")
+            made.append(("hiptensorDestroyTensorDescriptor", d))
+            return d
+
+        def modes(labels):
+            return (i32 * max(len(labels), 1))(*[ord(ch) for ch in labels])
+
+        c = torch.zeros_like(c_like)
+        da, db, dc = desc(a), desc(b), desc(c)
+        ma, mb, mc = modes(cfg["la"]), modes(cfg["lb"]), modes(cfg["lc"])
+        op, status = vp(), None
+        for comp in computes:
+            status = lib.hiptensorCreateContraction(handle, ctypes.byref(op), da, ma, 1, db, mb, 1,
+                                                    dc, mc, 1, dc, mc, comp)
+            if status == 0:
+                break
+        ck(status, "contraction descriptor")
+        made.append(("hiptensorDestroyOperationDescriptor", op))
+        pref = vp()
+        ck(lib.hiptensorCreatePlanPreference(handle, ctypes.byref(pref), -1, 0), "plan preference")
+        made.append(("hiptensorDestroyPlanPreference", pref))
+        ws = u64(0)
+        ck(lib.hiptensorEstimateWorkspaceSize(handle, op, pref, 2, ctypes.byref(ws)), "workspace")
+        plan = vp()
+        ck(lib.hiptensorCreatePlan(handle, ctypes.byref(plan), op, pref, ws.value), "plan")
+        made.append(("hiptensorDestroyPlan", plan))
+        work = torch.empty(max(int(ws.value), 1), dtype=torch.uint8, device=a.device)
+        one = (ctypes.c_float * 2)(1.0, 0.0) if cplx64 else (ctypes.c_double * 2)(1.0, 0.0)
+        stream = torch.cuda.current_stream()
+
+        def call():
+            ck(lib.hiptensorContract(handle, plan, ctypes.cast(one, vp), a.data_ptr(), b.data_ptr(),
+                                     ctypes.cast(one, vp), c.data_ptr(), c.data_ptr(),
+                                     work.data_ptr(), ws.value, stream.cuda_stream), "contract")
+
+        call()
+        mine = torch.zeros_like(c_like)
+        t1m.contract(a, cfg["la"], b, cfg["lb"], mine, cfg["lc"])
+        torch.cuda.synchronize()
+        rel = float(torch.linalg.vector_norm(c - mine) / torch.linalg.vector_norm(mine))
+        for _ in range(2):
+            call()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            call()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        fl, _ = flops(cfg)
+        return {"value": round(fl / (ms * 1e-3) / 1e12, 3), "unit": "TFLOP/s", "ms": round(ms, 4),
+                "workspace_bytes": int(ws.value), "rel_diff_vs_t1m": float(f"{rel:.3e}"),
+                "note": "libhiptensor contraction, default algorithm, same operands; comparison only"}
+    except RuntimeError as e:
+        return {"status": str(e)}
+    finally:
+        for f, h in reversed(made):
+            getattr(lib, f)(h)
+        lib.hiptensorDestroy(handle)
+
+
 def cpu_einsum(cfg, torch):
     """torch.einsum on the host, all threads, the full workload (one call
     after a warm-up): the library route on the CPU."""
@@ -325,6 +433,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c5"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-errors", action="store_true", help="skip the error report")
+    ap.add_argument("--no-library", action="store_true",
+                    help="skip the vendor GEMM and hipTensor comparisons")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -388,7 +498,9 @@ def main():
     if not args.no_errors:
         errs = rel_errors(t1m, torch, cfg, 4)
         line["rel_err_vs_fp64"] = {p: float(f"{e:.3e}") for p, e in errs.items()}
-    line["library_gemm"] = library_gemm(cfg, torch, a, b, torch.zeros_like(c), args.steps)
+    if not args.no_library:
+        line["library_gemm"] = library_gemm(cfg, torch, a, b, torch.zeros_like(c), args.steps)
+        line["hiptensor"] = hiptensor_contract(cfg, torch, t1m, a, b, c, args.steps)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg)
         nt = cpu_threads()

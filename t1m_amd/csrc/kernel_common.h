@@ -116,6 +116,34 @@ __device__ __forceinline__ void store_tile(bool atomic, Has has, Ok ok, Val val,
   }
 }
 
+// The two halves of store_tile's non-atomic path for a caller that loads
+// the C values early: all of them into cv, then cv + value stored.
+template <typename T, int NR, int NC, typename Has, typename Addr>
+__device__ __forceinline__ void load_c_tile(Has has, Addr addr, T (&cv)[NR][NC][4]) {
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (has(r, c))
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cv[r][c][q] = *addr(r, c, q);
+}
+
+template <typename T, int NR, int NC, typename Has, typename Ok, typename Val, typename Addr>
+__device__ __forceinline__ void add_store_c_tile(Has has, Ok ok, Val val, Addr addr,
+                                                 const T (&cv)[NR][NC][4]) {
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (has(r, c))
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const T v = val(r, c, q);
+          if (ok(r, c, q)) *addr(r, c, q) = cv[r][c][q] + v;
+        }
+}
+
 // C[idx] += v; atomically when several workgroups share the tile (split K).
 template <typename T>
 __device__ inline void accumulate(T* dst, T v, bool atomic) {

@@ -432,7 +432,7 @@ k_contract_bf16x3_batched(const KArgs<Idx>* __restrict__ inst,
 
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN>
 hipError_t launch_t(const GemmProblem& g, hipStream_t s) {
-  const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
+  const KArgs<Idx> a = make_args<Idx>(g, BM, BN, BK);
   const int64_t per = int64_t(a.tiles_m) * a.tiles_n * a.ksplit;
   const dim3 block(64 * WGM * WGN);
   const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;

@@ -225,7 +225,7 @@ struct RealCfg {
     });
   }
   static hipError_t launch(const GemmProblem& g, hipStream_t s) {
-    const KArgs<Idx> a = make_args<Idx>(g, BM, BN);
+    const KArgs<Idx> a = make_args<Idx>(g, BM, BN, BK);
     const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
     if (af && bf) return launch_one<true, true>(a, s);
     if (af) return launch_one<true, false>(a, s);

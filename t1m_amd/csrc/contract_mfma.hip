@@ -31,8 +31,10 @@
 // Wave layout: WGM x WGN waves.  A wave owns BM/WGM complex rows (MB blocks
 // of 8) and NBW = floor(BN/16 / WGN) blocks of 16 columns.  When BN/16 is
 // not a multiple of WGN, the one leftover 16-column block is shared: each
-// n-wave takes XB = ceil(MB/WGN) of its row blocks, reading those A
-// fragments from LDS by a runtime index.  For 144 x 144 tiles on 2 x 4 waves
+// n-wave takes XB = ceil(MB/WGN) of its row blocks, reusing the A operands
+// it reads for its full blocks (a wave-uniform branch per row block; +0.5
+// to +3.5 % over reading them again; the same change in the BF16x3 kernel,
+// without scheduling barriers, lost 8-13 %).  For 144 x 144 tiles on 2 x 4 waves
 // (2 waves per SIMD) a wave computes 18 + 3 MFMA tiles, 21 against an even
 // share of 20.25.
 #include "internal.h"
@@ -163,12 +165,6 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   const int am0 = wm * WTM + (l16 >> 1);
   const int bn0 = wn * NBW * 16 + l16;
   const int bnx = NBW * WGN * 16 + l16;
-  int xmb[XB_ALLOC];                 // shared-block row blocks of this wave
-#pragma unroll
-  for (int x = 0; x < XB_ALLOC; ++x) {
-    const int mb = wn * XB + x;
-    xmb[x] = mb < MB ? mb : MB - 1;
-  }
 
   // A operands: the two 1M operands of this lane's row parity (AEXP) or
   // the raw (re, im) pair.
@@ -201,7 +197,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   // half the stage's MFMAs.  The wave's priority is raised over the MFMA
   // stream and lowered for the writes (+1-2 %).
   constexpr int KCH = BK / 4;
-  constexpr int NG = MB + (XT ? XB : 0);     // row groups per chunk
+  constexpr int NG = MB;                     // row groups per chunk
   constexpr int TOT = KCH * NG;
   constexpr int LA = 2;
   struct BFr {
@@ -217,22 +213,25 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   };
   auto load_a = [&](int buf, int t) -> C2 {
     const int kc = t / NG, grp = t % NG;
-    const int q = kc * 4 + g;
-    if (grp < MB) return aread(buf, q, am0 + grp * 8);
-    return aread(buf, q, am0 + xmb[XT ? grp - MB : 0] * 8);
+    return aread(buf, kc * 4 + g, am0 + grp * 8);
   };
+  // The shared column block reuses the A operands of this wave's share of
+  // row blocks (grp / XB == wn, a wave-uniform branch) instead of reading
+  // them again.
   auto mma_grp = [&](int grp, const C2& a, const BFr& f) {
     T re, im;
     operands(a, re, im);
-    if (grp < MB) {
 #pragma unroll
-      for (int nb = 0; nb < NBW; ++nb) {
-        acc[grp][nb] = mfma16(f.b[nb].x, re, acc[grp][nb]);
-        acc[grp][nb] = mfma16(f.b[nb].y, im, acc[grp][nb]);
+    for (int nb = 0; nb < NBW; ++nb) {
+      acc[grp][nb] = mfma16(f.b[nb].x, re, acc[grp][nb]);
+      acc[grp][nb] = mfma16(f.b[nb].y, im, acc[grp][nb]);
+    }
+    if constexpr (XT) {
+      if (grp / XB == wn) {
+        Acc& ax = accx[grp % XB];
+        ax = mfma16(f.bx.x, re, ax);
+        ax = mfma16(f.bx.y, im, ax);
       }
-    } else if constexpr (XT) {
-      accx[grp - MB] = mfma16(f.bx.x, re, accx[grp - MB]);
-      accx[grp - MB] = mfma16(f.bx.y, im, accx[grp - MB]);
     }
   };
   auto compute_fine = [&](int buf, auto&& mid) {

@@ -34,7 +34,7 @@
 // n-wave takes XB = ceil(MB/WGN) of its row blocks, reusing the A operands
 // it reads for its full blocks (a wave-uniform branch per row block; +0.5
 // to +3.5 % over reading them again; the same change in the BF16x3 kernel,
-// without scheduling barriers, lost 8-13 %).  For 144 x 144 tiles on 2 x 4 waves
+// as a branch or with selects, lost 8-13 %).  For 144 x 144 tiles on 2 x 4 waves
 // (2 waves per SIMD) a wave computes 18 + 3 MFMA tiles, 21 against an even
 // share of 20.25.
 #include "internal.h"

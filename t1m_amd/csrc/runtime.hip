@@ -455,6 +455,46 @@ void order_by(std::vector<Mode>& modes, int which) {
   });
 }
 
+// Both operands gathered along k with different unit-stride k modes: a
+// traversal led by either one leaves the other strided (every lane of a
+// 16-k stage in its own cache line).  Splitting each one's fastest k mode
+// into 4 x rest and leading with both 4-blocks makes a stage read runs of
+// four consecutive elements from either operand.  The traversal order is
+// free: A's and B's maps are built from the same mode list.
+void block_k_order(std::vector<Mode>& k) {
+  auto fastest = [&](int which) {
+    int best = -1;
+    int64_t bs = INT64_MAX;
+    for (size_t i = 0; i < k.size(); ++i) {
+      if (k[i].dim <= 1) continue;
+      int64_t st = which == 0 ? k[i].sa : k[i].sb;
+      st = st < 0 ? -st : st;
+      if (st < bs) { bs = st; best = int(i); }
+    }
+    return best;
+  };
+  const int ia = fastest(0), ib = fastest(1);
+  if (ia < 0 || ib < 0 || ia == ib) return;
+  auto small = [](int64_t st) { return st >= -2 && st <= 2; };   // 4 elements within a line
+  if (!small(k[ia].sa) || !small(k[ib].sb)) return;
+  if (k[ia].dim % 4 != 0 || k[ib].dim % 4 != 0) return;
+  if (k.size() + 2 > size_t(T1M_MAX_RANK)) return;
+  auto split = [](const Mode& m, Mode& lo, Mode& hi) {
+    lo = m;
+    lo.dim = 4;
+    hi = m;
+    hi.dim = m.dim / 4;
+    hi.sa = 4 * m.sa; hi.sb = 4 * m.sb; hi.sc = 4 * m.sc;
+  };
+  Mode alo, ahi, blo, bhi;
+  split(k[ia], alo, ahi);
+  split(k[ib], blo, bhi);
+  std::vector<Mode> out = {alo, blo, ahi, bhi};
+  for (size_t i = 0; i < k.size(); ++i)
+    if (int(i) != ia && int(i) != ib) out.push_back(k[i]);
+  k.swap(out);
+}
+
 void fill_map(MapDesc& d, const std::vector<Mode>& modes, int which, bool out32,
               const void* out) {
   d.nmodes = int32_t(modes.size());
@@ -522,6 +562,7 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
   order_by(P.m, pl.a_mfast ? 0 : 2);
   order_by(P.k, pl.b_kfast ? 1 : (pl.a_mfast ? 1 : 0));
   order_by(P.n, pl.b_kfast ? 2 : 1);
+  if (!pl.a_mfast && pl.b_kfast) block_k_order(P.k);
 
   const int64_t lim = (int64_t(1) << 31) - 1;
   const int64_t offA = max_abs_offset(P.m, 0) + max_abs_offset(P.k, 0);

@@ -662,6 +662,28 @@ def test_batch_matches_single_calls():
     batch.close()
 
 
+@pytest.mark.parametrize("ext", [dict(a=5, b=8, c=7, d=12, e=6, f=5),
+                                 dict(a=9, b=4, c=3, d=20, e=3, f=7),
+                                 dict(a=40, b=16, c=2, d=24, e=33, f=2)])
+def test_both_operands_gathered_along_k(ext):
+    """A 'bdac' and B 'dbef': each operand's unit-stride mode is a different
+    contracted mode, so the planner splits both into 4-blocks and
+    interleaves them in the k traversal; every kernel against the oracle."""
+    rng = np.random.default_rng(23)
+    la, lb, lc = "bdac", "dbef", "acef"
+    for dtype in (np.complex64, np.complex128, np.float32, np.float64):
+        A = rand(rng, [ext[x] for x in la], dtype)
+        B = rand(rng, [ext[x] for x in lb], dtype)
+        C0 = rand(rng, [ext[x] for x in lc], dtype)
+        want = oracle_einsum(A, la, B, lb, C0, lc)
+        for kern, prec, tolkey in ALL_KERNELS[dtype]:
+            with forced(kern):
+                got = run(A, la, B, lb, C0, lc, precision=prec)
+            err = rel_err(got - C0, want - C0)
+            print(f"  {ext} {np.dtype(dtype).name} {kern} rel={err:.2e}")
+            assert err < TOL[tolkey], (ext, dtype, kern, err)
+
+
 def test_aliased_outputs_add_every_contribution():
     """C views whose index tuples share addresses (a zero stride, interleaved
     strides) get every contribution, as a serial loop over the outputs would

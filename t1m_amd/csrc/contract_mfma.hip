@@ -31,12 +31,12 @@
 // Wave layout: WGM x WGN waves.  A wave owns BM/WGM complex rows (MB blocks
 // of 8) and NBW = floor(BN/16 / WGN) blocks of 16 columns.  When BN/16 is
 // not a multiple of WGN, the one leftover 16-column block is shared: each
-// n-wave takes XB = ceil(MB/WGN) of its row blocks, reusing the A operands
+// n-wave takes MB/WGN or one more of its row blocks, reusing the A operands
 // it reads for its full blocks (a wave-uniform branch per row block; +0.5
 // to +3.5 % over reading them again; the same change in the BF16x3 kernel,
 // as a branch or with selects, lost 8-13 %).  For 144 x 144 tiles on 2 x 4 waves
-// (2 waves per SIMD) a wave computes 18 + 3 MFMA tiles, 21 against an even
-// share of 20.25.
+// (2 waves per SIMD) a SIMD computes 41 MFMA tiles per k chunk against an
+// even share of 40.5.
 #include "internal.h"
 #include "kernel_common.h"
 
@@ -105,6 +105,15 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int wm = wid % WGM, wn = wid / WGM;
+  // Shared column block: n-wave wn takes row blocks [xlo, xhi), MB / WGN
+  // each and one more for the first MB % WGN waves.  Waves wn and wn + 2
+  // share a SIMD (wave id = wm + WGM wn, WGM = 2), so the extra blocks land
+  // on different SIMDs: 41 + 41 + 40 + 40 MFMA tiles per SIMD pair at
+  // 144 x 144, where ceil(MB / WGN) per wave gave 42 + 42 + 39 + 39.  A
+  // block mb sits in accx[mb % XB] (a range of at most XB blocks has
+  // distinct residues).
+  const int xlo = wn * (MB / WGN) + (wn < MB % WGN ? wn : MB % WGN);
+  const int xhi = xlo + MB / WGN + (wn < MB % WGN ? 1 : 0);
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
   const int64_t M = p.M, N = p.N;
   const int K = kend;     // contracted range [kbeg, kend) of this workgroup
@@ -216,7 +225,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
     return aread(buf, kc * 4 + g, am0 + grp * 8);
   };
   // The shared column block reuses the A operands of this wave's share of
-  // row blocks (grp / XB == wn, a wave-uniform branch) instead of reading
+  // row blocks (xlo <= grp < xhi, a wave-uniform branch) instead of reading
   // them again.
   auto mma_grp = [&](int grp, const C2& a, const BFr& f) {
     T re, im;
@@ -227,7 +236,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
       acc[grp][nb] = mfma16(f.b[nb].y, im, acc[grp][nb]);
     }
     if constexpr (XT) {
-      if (grp / XB == wn) {
+      if (grp >= xlo && grp < xhi) {
         Acc& ax = accx[grp % XB];
         ax = mfma16(f.bx.x, re, ax);
         ax = mfma16(f.bx.y, im, ax);
@@ -284,9 +293,10 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   auto offsets = [&] {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-      const int blk = r < MB ? r : wn * XB + (r - MB);
-      const int64_t i = i0 + am0 + (r < MB ? r : (blk < MB ? blk : MB - 1)) * 8;
-      rok[r] = i < M && (r < MB || blk < MB);
+      // shared slot x = r - MB holds block xlo + ((x - xlo) mod XB), if < xhi
+      const int blk = r < MB ? r : xlo + ((r - MB) - xlo % XB_ALLOC + XB_ALLOC) % XB_ALLOC;
+      const int64_t i = i0 + am0 + (blk < MB ? blk : MB - 1) * 8;
+      rok[r] = i < M && (r < MB || blk < xhi);
       rofs[r] = rok[r] ? p.rsC[i] : Idx(0);
     }
 #pragma unroll

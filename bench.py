@@ -9,6 +9,11 @@ FP32-MFMA path.  The same inputs then run on the BF16x3 path, and both
 paths' relative Frobenius errors are measured on log-uniform-magnitude inputs
 against a complex128 reference.
 
+Timing: `value` is flop x steps over the host wall time of `steps` calls
+issued back to back between two synchronizations; `roofline.achieved` and
+the kernel_ms figures come from a second pass of the same calls, each
+bracketed by HIP events on the launch stream (see timed_region).
+
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -74,23 +79,41 @@ def make_inputs(cfg, kind, seed, torch):
     return one(cfg["la"]), one(cfg["lb"]), one(cfg["lc"]) * 0
 
 
-def time_path(t1m, torch, a, b, c, cfg, precision, steps, warmup):
-    stream = torch.cuda.current_stream()
-    for _ in range(max(warmup, 1)):   # the first call autotunes the tile shape
-        t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision,
-                     synchronize=False, autotune=True)
+def timed_region(torch, stream, fn, steps, warmup):
+    """`warmup` untimed calls, then two passes of `steps` calls each:
+    (1) the timed region proper -- synchronize, host clock, `steps` calls
+    back to back, synchronize -- giving the whole-job wall time; (2) the same
+    calls each bracketed by HIP events on the launch stream, giving every
+    launch's duration.  The per-launch events are kept out of (1): each
+    event record puts a marker between two launches that idles the GPU for
+    a few microseconds, which is measurement, not workload."""
+    for _ in range(max(warmup, 1)):
+        fn()
     torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(steps)]
     t0 = time.perf_counter()
-    for s, e in evs:
-        s.record(stream)
-        t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision,
-                     synchronize=False)
-        e.record(stream)
+    for _ in range(steps):
+        fn()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = [s.elapsed_time(e) for s, e in evs]
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    for s_, e_ in evs:
+        s_.record(stream)
+        fn()
+        e_.record(stream)
+    torch.cuda.synchronize()
+    return wall, [s_.elapsed_time(e_) for s_, e_ in evs]
+
+
+def time_path(t1m, torch, a, b, c, cfg, precision, steps, warmup):
+    stream = torch.cuda.current_stream()
+    # the first call autotunes the tile shape
+    t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision,
+                 synchronize=False, autotune=True)
+    wall, kern_ms = timed_region(
+        torch, stream,
+        lambda: t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision,
+                             synchronize=False), steps, warmup)
     kernel = t1m.plan_kernel(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision)
     return wall, kern_ms, kernel
 
@@ -385,19 +408,8 @@ def bench_c5(args, torch, t1m):
     build_s = time.perf_counter() - t0
 
     def timed(fn):
-        for _ in range(max(args.warmup, 1)):
-            fn()
-        torch.cuda.synchronize()
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
-        t = time.perf_counter()
-        for s_, e_ in ev:
-            s_.record(stream)
-            fn()
-            e_.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t
-        return wall, sum(s_.elapsed_time(e_) for s_, e_ in ev) / len(ev)
+        wall, kern = timed_region(torch, stream, fn, args.steps, args.warmup)
+        return wall, sum(kern) / len(kern)
 
     wall_f, kf = timed(lambda: fwd.launch(stream))
     wall_v, kv = timed(lambda: (vja.launch(stream), vjb.launch(stream)))

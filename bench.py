@@ -81,20 +81,16 @@ def make_inputs(cfg, kind, seed, torch):
 
 def timed_region(torch, stream, fn, steps, warmup):
     """`warmup` untimed calls, then two passes of `steps` calls each:
-    (1) the timed region proper -- synchronize, host clock, `steps` calls
-    back to back, synchronize -- giving the whole-job wall time; (2) the same
-    calls each bracketed by HIP events on the launch stream, giving every
-    launch's duration.  The per-launch events are kept out of (1): each
-    event record puts a marker between two launches that idles the GPU for
-    a few microseconds, which is measurement, not workload."""
+    (1) every call bracketed by HIP events on the launch stream, giving each
+    launch's duration (and warming the clocks further); (2) the timed region
+    proper -- synchronize, host clock, `steps` calls back to back,
+    synchronize -- giving the whole-job wall time.  The per-launch events
+    are kept out of (2): each event record puts a marker between two
+    launches that idles the GPU for a few microseconds, which is
+    measurement, not workload."""
     for _ in range(max(warmup, 1)):
         fn()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        fn()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
     for s_, e_ in evs:
@@ -102,6 +98,11 @@ def timed_region(torch, stream, fn, steps, warmup):
         fn()
         e_.record(stream)
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
     return wall, [s_.elapsed_time(e_) for s_, e_ in evs]
 
 
@@ -427,7 +428,8 @@ def bench_c5(args, torch, t1m):
         "roofline": {"bound": "mfma", "achieved": round(fwd_flop / (kf * 1e-3) / 1e12, 3),
                      "peak": PEAK["fp32"], "unit": "TFLOP/s",
                      "frac": round(fwd_flop / (kf * 1e-3) / 1e12 / PEAK["fp32"], 4),
-                     "traffic": None, "kernel_ms": round(kf, 4)},
+                     "traffic": traffic_bytes("c5", "c64_f32mfma_64x64_batched_fwd"),
+                     "kernel_ms": round(kf, 4)},
         "vjp": {"value": round(2 * fwd_flop * args.steps / wall_v / 1e12, 3), "unit": "TFLOP/s",
                 "ms_per_step": round(wall_v / args.steps * 1e3, 4),
                 "flop_per_step": 2 * fwd_flop},

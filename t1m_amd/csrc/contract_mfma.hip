@@ -545,7 +545,9 @@ template <typename Idx, typename F>
 auto with_1m_config(int32_t kernel, F&& f) {
   switch (kernel) {
     case K_C64_F32_144:   // batches run the small tiles only
-    case K_C64_F32_64: return f(Batched1M<float, Idx, 64, 64, 16, 2, 2>());
+    // Stages of 8 k: half the LDS and 160 registers, 3 workgroups per CU
+    // instead of 2 (+2 % on 512 MPS contractions; items are short in k).
+    case K_C64_F32_64: return f(Batched1M<float, Idx, 64, 64, 8, 2, 2>());
     case K_C128_F64_64: return f(Batched1M<double, Idx, 64, 64, 16, 2, 2>());
     default: return f(Batched1M<double, Idx, 32, 32, 16, 2, 2>());
   }

@@ -11,8 +11,8 @@ against a complex128 reference.
 
 Timing: `value` is flop x steps over the host wall time of `steps` calls
 issued back to back between two synchronizations; `roofline.achieved` and
-the kernel_ms figures come from a second pass of the same calls, each
-bracketed by HIP events on the launch stream (see timed_region).
+the kernel_ms figures come from a pass of the same calls just before it,
+each bracketed by HIP events on the launch stream (see timed_region).
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """

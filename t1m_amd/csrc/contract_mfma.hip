@@ -477,7 +477,9 @@ template <typename Idx>
 hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
   switch (kernel) {
     case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, 16, 2, 4>(g, s);
-    case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 16, 2, 2>(g, s);
+    // 8-k stages: 3 workgroups per CU (+5 % and +13 % on two 48^4 / 24^4
+    // layouts, -1 % on a third).
+    case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 8, 2, 2>(g, s);
     // 4 x 2 waves (16 x 32 complex per wave, 2 waves per SIMD): +23 % over
     // 2 x 2 at 1024 x 1024 x 256, +3 % at 2304^3.
     case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16, 4, 2>(g, s);

@@ -662,15 +662,26 @@ def test_batch_matches_single_calls():
     batch.close()
 
 
-@pytest.mark.parametrize("ext", [dict(a=5, b=8, c=7, d=12, e=6, f=5),
-                                 dict(a=9, b=4, c=3, d=20, e=3, f=7),
-                                 dict(a=40, b=16, c=2, d=24, e=33, f=2)])
-def test_both_operands_gathered_along_k(ext):
-    """A 'bdac' and B 'dbef': each operand's unit-stride mode is a different
-    contracted mode, so the planner splits both into 4-blocks and
-    interleaves them in the k traversal; every kernel against the oracle."""
-    rng = np.random.default_rng(23)
-    la, lb, lc = "bdac", "dbef", "acef"
+BLOCKED_LAYOUTS = [
+    # k: A and B gathered along k, unit strides on different contracted modes
+    (("bdac", "dbef", "acef"), dict(a=5, b=8, c=7, d=12, e=6, f=5)),
+    (("bdac", "dbef", "acef"), dict(a=9, b=4, c=3, d=20, e=3, f=7)),
+    (("bdac", "dbef", "acef"), dict(a=40, b=16, c=2, d=24, e=33, f=2)),
+    # m: A gathered along m with unit stride on e, C's on a
+    (("efbad", "cf", "abcde"), dict(a=8, b=4, c=3, d=4, e=12, f=4)),
+    # n: B gathered along n with unit stride on d, C's on c
+    (("ka", "dck", "cad"), dict(a=8, k=12, c=8, d=16)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(BLOCKED_LAYOUTS)))
+def test_blocked_bundle_orders(case):
+    """Layouts where two tensors of one bundle have their unit stride on
+    different modes, so the planner splits both into 4-blocks and
+    interleaves them in the traversal (k: A and B; m: C and A; n: C and
+    B); every kernel against the oracle."""
+    (la, lb, lc), ext = BLOCKED_LAYOUTS[case]
+    rng = np.random.default_rng(23 + case)
     for dtype in (np.complex64, np.complex128, np.float32, np.float64):
         A = rand(rng, [ext[x] for x in la], dtype)
         B = rand(rng, [ext[x] for x in lb], dtype)
@@ -680,8 +691,8 @@ def test_both_operands_gathered_along_k(ext):
             with forced(kern):
                 got = run(A, la, B, lb, C0, lc, precision=prec)
             err = rel_err(got - C0, want - C0)
-            print(f"  {ext} {np.dtype(dtype).name} {kern} rel={err:.2e}")
-            assert err < TOL[tolkey], (ext, dtype, kern, err)
+            print(f"  {la},{lb}->{lc} {ext} {np.dtype(dtype).name} {kern} rel={err:.2e}")
+            assert err < TOL[tolkey], (la, lb, lc, ext, dtype, kern, err)
 
 
 def test_aliased_outputs_add_every_contribution():

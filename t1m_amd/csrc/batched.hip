@@ -59,7 +59,10 @@ int cu_count_batch() {
 int batch_kernel(int dtype, int precision) {
   if (const char* f = getenv("T1M_FORCE_KERNEL")) {
     const int k = atoi(f);
-    if (kernel_fits(k, dtype, precision) && k != K_C64_F32_144 && k != K_C64_BF16X3_144) return k;
+    // batches run the small square tiles only
+    if (kernel_fits(k, dtype, precision) && k != K_C64_F32_144 && k != K_C64_BF16X3_144 &&
+        k != K_C64_F32_THIN && k != K_C128_F64_THIN)
+      return k;
   }
   if (precision == T1M_PRECISION_SIMPLE) return K_SIMPLE;
   switch (dtype) {

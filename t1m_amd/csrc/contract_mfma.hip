@@ -484,6 +484,9 @@ hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
     // 2 x 2 at 1024 x 1024 x 256, +3 % at 2304^3.
     case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16, 4, 2>(g, s);
     case K_C128_F64_32: return launch_1m<double, Idx, 32, 32, 16, 2, 2>(g, s);
+    // Thin C (one 16-column block): 4 waves stacked along m.
+    case K_C64_F32_THIN: return launch_1m<float, Idx, 128, 16, 16, 4, 1>(g, s);
+    case K_C128_F64_THIN: return launch_1m<double, Idx, 128, 16, 16, 4, 1>(g, s);
     default: return launch_simple<Idx>(g, s);
   }
 }
@@ -500,6 +503,8 @@ const KernelInfo kInfo[K_COUNT] = {
     {"f32mfma_64x64", 64, 64},
     {"f64mfma_64x64", 64, 64},
     {"f64mfma_32x32", 32, 32},
+    {"c64_f32mfma_128x16", 128, 16},
+    {"c128_f64mfma_128x16", 128, 16},
 };
 
 }  // namespace
@@ -546,7 +551,8 @@ struct Batched1M {
 template <typename Idx, typename F>
 auto with_1m_config(int32_t kernel, F&& f) {
   switch (kernel) {
-    case K_C64_F32_144:   // batches run the small tiles only
+    case K_C64_F32_144:   // batches run the small square tiles only
+    case K_C64_F32_THIN:
     // Stages of 8 k: half the LDS and 160 registers, 3 workgroups per CU
     // instead of 2 (+2 % on 512 MPS contractions; items are short in k).
     case K_C64_F32_64: return f(Batched1M<float, Idx, 64, 64, 8, 2, 2>());

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -196,9 +197,9 @@ std::vector<int> kernel_candidates(int dtype, int precision) {
   switch (dtype) {
     case T1M_COMPLEX64:
       if (precision == T1M_PRECISION_BF16X3) return {K_C64_BF16X3_144, K_C64_BF16X3_64};
-      return {K_C64_F32_144, K_C64_F32_64};
+      return {K_C64_F32_144, K_C64_F32_64, K_C64_F32_THIN};
     case T1M_COMPLEX128:
-      return {K_C128_F64_64, K_C128_F64_32};
+      return {K_C128_F64_64, K_C128_F64_32, K_C128_F64_THIN};
     case T1M_FLOAT32:
       return {K_F32_128, K_F32_64};
     case T1M_FLOAT64:
@@ -215,10 +216,15 @@ bool kernel_fits(int kernel, int dtype, int precision) {
   return false;
 }
 
-// Static rule, used when no tuning result is known: the large tiles (one
-// workgroup of 2 waves per SIMD per CU) once they nearly fill the CUs,
-// otherwise the small tiles (several workgroups per CU).  Calibrated on the
-// 48^4 and 256x4x256 workloads.
+// Static rule, used when no tuning result is known.  complex<float>: the
+// thin 128 x 16 tiles for C at most 16 wide; else the 144 x 144 tiles (one
+// workgroup of 2 waves per SIMD per CU) once they nearly fill the CUs and a
+// relative-time model favours them: 1 / (rate x tile utilisation x wave
+// efficiency), rates 0.72 (144) and 0.57 (64) of peak for FP32 and 1.0 /
+// 0.6 for BF16x3 as measured on 48^4, utilisation = the useful fraction of
+// the padded tile grid, wave efficiency = tiles / (rounds x CUs) for the
+// one-per-CU large tiles (the small ones run 3 per CU and fill in).
+// Calibrated on the 48^4, 256x4x256 and reference-suite contractions.
 int choose_kernel(const Problem& P, int precision) {
   if (P.K >= (int64_t(1) << 31)) return K_SIMPLE;     // MFMA kernels use int k
   if (const char* f = getenv("T1M_FORCE_KERNEL")) {   // tests / tuning; ignored
@@ -231,11 +237,22 @@ int choose_kernel(const Problem& P, int precision) {
   };
   switch (P.dtype) {
     case T1M_COMPLEX64: {
-      const bool big = tiles(144, 144) >= 0.75 * cu_count();
-      if (precision == T1M_PRECISION_BF16X3) return big ? K_C64_BF16X3_144 : K_C64_BF16X3_64;
+      const bool bf = precision == T1M_PRECISION_BF16X3;
+      if (!bf && P.N <= 16) return K_C64_F32_THIN;   // one 16-column block
+      auto util = [&](int bm, int bn) {
+        return double(P.M) / double((P.M + bm - 1) / bm * bm) *
+               (double(P.N) / double((P.N + bn - 1) / bn * bn));
+      };
+      const double cus = cu_count(), t144 = tiles(144, 144);
+      const double wave = t144 / (std::ceil(t144 / cus) * cus);
+      const double r144 = bf ? 1.0 : 0.72, r64 = bf ? 0.6 : 0.57;
+      const bool big = t144 >= 0.75 * cus &&
+                       r144 * util(144, 144) * wave >= r64 * util(64, 64);
+      if (bf) return big ? K_C64_BF16X3_144 : K_C64_BF16X3_64;
       return big ? K_C64_F32_144 : K_C64_F32_64;
     }
     case T1M_COMPLEX128:
+      if (P.N <= 16) return K_C128_F64_THIN;
       return tiles(64, 64) >= cu_count() ? K_C128_F64_64 : K_C128_F64_32;
     case T1M_FLOAT32:
       return tiles(128, 128) >= cu_count() ? K_F32_128 : K_F32_64;
@@ -357,6 +374,8 @@ int autotune_plan(Plan& pl, const GemmProblem& g0, hipStream_t stream) {
   int best = pl.kernel;
   float best_ms = 1e30f;
   for (int k : cands) {
+    // thin tiles only where C is narrow (elsewhere they cannot win)
+    if ((k == K_C64_F32_THIN || k == K_C128_F64_THIN) && pl.N > 64) continue;
     hipError_t e = launch_contract(g, k, stream);          // warm-up
     float kbest = 1e30f;
     for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {

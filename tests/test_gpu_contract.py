@@ -208,9 +208,11 @@ RAGGED = [
 ]
 
 KERNELS_C64 = [("c64_f32mfma_144x144", "native", "fp32"), ("c64_f32mfma_64x64", "native", "fp32"),
+               ("c64_f32mfma_128x16", "native", "fp32"),
                ("c64_bf16x3_144x144", "bf16x3", "bf16x3"), ("c64_bf16x3_64x64", "bf16x3", "bf16x3"),
                ("simple", "native", "simple32")]
 KERNELS_C128 = [("c128_f64mfma_64x64", "native", "fp64"), ("c128_f64mfma_32x32", "native", "fp64"),
+                ("c128_f64mfma_128x16", "native", "fp64"),
                 ("simple", "native", "fp64")]
 
 

@@ -814,3 +814,27 @@ def test_fuzz_random_contractions(seed):
             err = rel_err(got - C0, want - C0)
             print(f"  {la},{lb}->{lc} {ext} {np.dtype(dtype).name} {kern} rel={err:.2e}")
             assert err < TOL[tolkey], (la, lb, lc, ext, kern, err)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_fuzz_blocked_orders(seed):
+    """Random label sets with extents in {4, 8, 12, 16} and randomly
+    permuted storage orders, so that the planner's 4-block interleaving of
+    k, m and n traversals (unit strides of two tensors on different modes)
+    is exercised often; every kernel of each dtype against the oracle."""
+    rng = np.random.default_rng(2000 + seed)
+    for case in range(4):
+        la, lb, lc, _ = _random_problem(rng)
+        ext = {ch: int(rng.choice([4, 8, 12, 16])) for ch in la + lb + lc}
+        dtype = [np.complex64, np.complex128, np.float32, np.float64][int(rng.integers(0, 4))]
+        A = rand(rng, [ext[x] for x in la], dtype)
+        B = rand(rng, [ext[x] for x in lb], dtype)
+        C0 = rand(rng, [ext[x] for x in lc], dtype)
+        want = oracle_einsum(A, la, B, lb, C0, lc)
+        for kern, prec, tolkey in ALL_KERNELS[dtype]:
+            a, b, c = _random_layout(rng, A), _random_layout(rng, B), _random_layout(rng, C0)
+            with forced(kern):
+                t1m_amd.contract(a, la, b, lb, c, lc, precision=prec)
+            err = rel_err(c.cpu().numpy() - C0, want - C0)
+            print(f"  {la},{lb}->{lc} {ext} {np.dtype(dtype).name} {kern} rel={err:.2e}")
+            assert err < TOL[tolkey], (la, lb, lc, ext, kern, err)

@@ -838,3 +838,22 @@ def test_fuzz_blocked_orders(seed):
             err = rel_err(c.cpu().numpy() - C0, want - C0)
             print(f"  {la},{lb}->{lc} {ext} {np.dtype(dtype).name} {kern} rel={err:.2e}")
             assert err < TOL[tolkey], (la, lb, lc, ext, kern, err)
+
+
+def test_static_tile_rule():
+    """Without tuning: thin C (at most 16 wide) takes the 128 x 16 tiles, a
+    48^4 contraction the 144 x 144 tiles, a 324-tile GEMM (1.27 rounds of
+    144-tiles) and a 36-row C the 64 x 64 tiles; FP64 thin C its thin
+    tiles."""
+    def kern(la, lb, lc, ext, dtype=torch.complex64, precision="native"):
+        mk = lambda lab: torch.zeros([ext[x] for x in lab], dtype=dtype, device="cuda")
+        return t1m_amd.plan_kernel(mk(la), la, mk(lb), lb, mk(lc), lc, precision=precision)
+    with forced(None):
+        assert kern("efbad", "cf", "abcde", dict(a=24, b=16, c=12, d=16, e=24, f=16)) == "c64_f32mfma_128x16"
+        assert kern("efbad", "cf", "abcde", dict(a=24, b=16, c=12, d=16, e=24, f=16),
+                    torch.complex128) == "c128_f64mfma_128x16"
+        assert kern("abcd", "dbef", "acef", {ch: 48 for ch in "abcdef"}) == "c64_f32mfma_144x144"
+        assert kern("abcd", "dbef", "acef", {ch: 48 for ch in "abcdef"},
+                    precision="bf16x3") == "c64_bf16x3_144x144"
+        assert kern("ac", "cb", "ab", dict(a=2568, b=2560, c=64)) == "c64_f32mfma_64x64"
+        assert kern("ea", "ebcd", "abcd", dict(a=36, b=36, c=36, d=36, e=36)) == "c64_f32mfma_64x64"

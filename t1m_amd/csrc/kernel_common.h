@@ -161,13 +161,9 @@ __device__ inline void accumulate(T* dst, T v, bool atomic) {
 constexpr int kXcds = 8;
 constexpr int kGroupM = 4;
 
-#ifndef T1M_XCD_REMAP
-#define T1M_XCD_REMAP 1
-#endif
-
 __device__ inline void tile_coords_of(int w, int tiles_m, int tiles_n, int& tm, int& tn) {
   const int nwg = tiles_m * tiles_n;
-  if (T1M_XCD_REMAP && nwg % kXcds == 0) w = (w % kXcds) * (nwg / kXcds) + w / kXcds;
+  if (nwg % kXcds == 0) w = (w % kXcds) * (nwg / kXcds) + w / kXcds;
   const int band = kGroupM * tiles_n;
   const int b0 = w / band * kGroupM;
   const int rows = (tiles_m - b0) < kGroupM ? (tiles_m - b0) : kGroupM;

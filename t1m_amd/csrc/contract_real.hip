@@ -4,9 +4,11 @@
 // bli/mappings.h:91-99) on gfx950 matrix cores.
 //
 // Same structure as the complex kernels (contract_mfma.hip): one workgroup
-// per BM x BN tile of C, strided gathers into double-buffered LDS, MFMA
-// fragments read one 4-k chunk ahead, scatter-accumulate epilogue through
-// the scatter vectors.  Without 1M the MFMA is the plain real product: in
+// per BM x BN tile of C, strided gathers into double-buffered, XOR-swizzled
+// LDS, B fragments one 4-k chunk and A operands two row blocks ahead of
+// their MFMAs, the next stage's LDS writes in the middle of the stage,
+// grouped scatter-accumulate epilogue through the scatter vectors (the
+// pipeline and swizzle: +4..14 % over chunk-wise fragments and plain rows).  Without 1M the MFMA is the plain real product: in
 // the transposed form the MFMA rows are 16 columns j of C and its columns
 // are 16 rows i, so lane l holds C[i = l % 16][.] and 16 lanes write 16
 // consecutive rows.  v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64, one MFMA
@@ -32,10 +34,21 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
   constexpr bool XT = NBT % WGN != 0;
   constexpr int XB = XT ? (MB + WGN - 1) / WGN : 0;
   constexpr int XB_ALLOC = XT ? XB : 1;
-  constexpr int PAD = sizeof(T) == 4 ? 16 : 8;
-  constexpr int LDA = BM + PAD, LDB = BN + PAD;
+  // LDS rows of BX + PAD elements, PAD making the row pitch 16 words past a
+  // multiple of 64 words (bank = 16 k + x for 4-byte elements), and element
+  // (k, x) stored at column x ^ sw(k), sw(k) = 4 (k/4 mod 4) (4-byte) or
+  // 2 (k/4 mod 4) (8-byte): a gather along k (4 or 2 rows x 16 k per write)
+  // then hits every bank once, and the MFMA reads (16 consecutive x of one
+  // aligned group, 4 k) stay a permutation of one bank sweep.
+  constexpr int EW = int(sizeof(T)) / 4;             // words per element
+  constexpr int PADA = ((16 / EW - BM % (64 / EW)) % (64 / EW) + 64 / EW) % (64 / EW);
+  constexpr int PADB = ((16 / EW - BN % (64 / EW)) % (64 / EW) + 64 / EW) % (64 / EW);
+  constexpr int LDA = BM + PADA, LDB = BN + PADB;
+  static_assert((LDA * EW) % 64 == 16 && (LDB * EW) % 64 == 16, "row pitch");
   static_assert(WTM % 16 == 0 && BN % 16 == 0, "tile shape");
   static_assert(NBT - NBW * WGN <= 1, "at most one shared column block");
+  static_assert(BK % 4 == 0, "k chunk");
+  auto sw = [](int k) { return ((k >> 2) & 3) * (4 / EW); };
 
   __shared__ T sA[2][BK][LDA];
   __shared__ T sB[2][BK][LDB];
@@ -43,6 +56,11 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int wm = wid % WGM, wn = wid / WGM;
+  // Shared column block: n-wave wn takes row blocks [xlo, xhi) (MB / WGN
+  // each, one more for the first MB % WGN waves); block mb sits in
+  // accx[mb % XB].
+  const int xlo = wn * (MB / WGN) + (wn < MB % WGN ? wn : MB % WGN);
+  const int xhi = xlo + MB / WGN + (wn < MB % WGN ? 1 : 0);
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
   const int64_t M = p.M, N = p.N;
   const int K = kend;     // contracted range [kbeg, kend) of this workgroup
@@ -63,12 +81,14 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
 #pragma unroll
     for (int u = 0; u < GA::E; ++u) {
       const unsigned e = GA::elem(utid, u);
-      if (GA::valid(utid, u)) sA[buf][GA::kc(e)][GA::xc(e)] = ga.val[u];
+      const int k = GA::kc(e);
+      if (GA::valid(utid, u)) sA[buf][k][GA::xc(e) ^ sw(k)] = ga.val[u];
     }
 #pragma unroll
     for (int u = 0; u < GB::E; ++u) {
       const unsigned e = GB::elem(utid, u);
-      if (GB::valid(utid, u)) sB[buf][GB::kc(e)][GB::xc(e)] = gb.val[u];
+      const int k = GB::kc(e);
+      if (GB::valid(utid, u)) sB[buf][k][GB::xc(e) ^ sw(k)] = gb.val[u];
     }
   };
 
@@ -86,52 +106,65 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
   const int am0 = wm * WTM + l16;
   const int bn0 = wn * NBW * 16 + l16;
   const int bnx = NBW * WGN * 16 + l16;
-  int xmb[XB_ALLOC];
-#pragma unroll
-  for (int x = 0; x < XB_ALLOC; ++x) {
-    const int mb = wn * XB + x;
-    xmb[x] = mb < MB ? mb : MB - 1;
-  }
 
-  struct Frag {
-    T a[MB];
+  // Operand pipeline (as in the complex kernels): B fragments one 4-k
+  // chunk ahead, A operands LA row blocks ahead of their MFMAs, scheduling
+  // barriers keeping the order, the next stage's LDS writes after half the
+  // stage, wave priority raised over the MFMA stream.  The shared column
+  // block reuses the wave's A operands under a wave-uniform branch.
+  constexpr int KCH = BK / 4;
+  constexpr int TOT = KCH * MB;
+  constexpr int LA = 2;
+  struct BFr {
     T b[NBW];
-    T ax[XB_ALLOC];
     T bx;
   };
-  auto load_frag = [&](int buf, int kc, Frag& f) {
+  auto load_b = [&](int buf, int kc, BFr& f) {
     const int q = kc * 4 + g;
+    const int s = sw(q);
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) f.a[mb] = sA[buf][q][am0 + mb * 16];
+    for (int nb = 0; nb < NBW; ++nb) f.b[nb] = sB[buf][q][(bn0 + nb * 16) ^ s];
+    if constexpr (XT) f.bx = sB[buf][q][bnx ^ s];
+  };
+  auto load_a = [&](int buf, int t) -> T {
+    const int kc = t / MB, grp = t % MB;
+    const int q = kc * 4 + g;
+    return sA[buf][q][(am0 + grp * 16) ^ sw(q)];
+  };
+  auto mma_grp = [&](int grp, T a, const BFr& f) {
 #pragma unroll
-    for (int nb = 0; nb < NBW; ++nb) f.b[nb] = sB[buf][q][bn0 + nb * 16];
+    for (int nb = 0; nb < NBW; ++nb) acc[grp][nb] = mfma16(f.b[nb], a, acc[grp][nb]);
     if constexpr (XT) {
-      f.bx = sB[buf][q][bnx];
-#pragma unroll
-      for (int x = 0; x < XB; ++x) f.ax[x] = sA[buf][q][am0 + xmb[x] * 16];
+      if (grp >= xlo && grp < xhi) {
+        Acc& ax = accx[grp % XB];
+        ax = mfma16(f.bx, a, ax);
+      }
     }
   };
-  auto mma = [&](const Frag& f) {
+  auto compute = [&](int buf, auto&& mid) {
+    BFr fb[2];
+    T aw[LA + 1];
+    __builtin_amdgcn_s_setprio(1);
+    load_b(buf, 0, fb[0]);
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
+    for (int t = 0; t < LA; ++t) aw[t] = load_a(buf, t);
 #pragma unroll
-      for (int nb = 0; nb < NBW; ++nb) acc[mb][nb] = mfma16(f.b[nb], f.a[mb], acc[mb][nb]);
-    if constexpr (XT) {
-#pragma unroll
-      for (int x = 0; x < XB; ++x) accx[x] = mfma16(f.bx, f.ax[x], accx[x]);
-    }
-  };
-  auto compute = [&](int buf) {
-    constexpr int KCH = BK / 4;
-    Frag f[2];
-    load_frag(buf, 0, f[0]);
-#pragma unroll
-    for (int kc = 0; kc < KCH; ++kc) {
-      if (kc + 1 < KCH) load_frag(buf, kc + 1, f[(kc + 1) & 1]);
+    for (int t = 0; t < TOT; ++t) {
+      const int kc = t / MB, grp = t % MB;
+      if (grp == 0 && kc + 1 < KCH) load_b(buf, kc + 1, fb[(kc + 1) & 1]);
+      if (t + LA < TOT) aw[(t + LA) % (LA + 1)] = load_a(buf, t + LA);
       __builtin_amdgcn_sched_barrier(0);
-      mma(f[kc & 1]);
+      mma_grp(grp, aw[t % (LA + 1)], fb[kc & 1]);
       __builtin_amdgcn_sched_barrier(0);
+      if (t == TOT / 2 - 1) {
+        __builtin_amdgcn_s_setprio(0);
+        mid();
+        __builtin_amdgcn_s_setprio(1);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
+    __builtin_amdgcn_s_setprio(0);
+    if constexpr (TOT < 2) mid();
   };
 
   const int nst = (K - kbeg + BK - 1) / BK;
@@ -148,11 +181,10 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
     gb.load_data(B);
     ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
     gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
-    compute(s & 1);
-    swrite((s + 1) & 1, kbeg + (s + 1) * BK);
+    compute(s & 1, [&] { swrite((s + 1) & 1, kbeg + (s + 1) * BK); });
     __syncthreads();
   }
-  compute((nst - 1) & 1);
+  compute((nst - 1) & 1, [] {});
 
   // Epilogue: this lane holds rows i0 + am0 + 16 r' (one per row block)
   // times 4 columns per 16-column block; offsets first, then store_tile.
@@ -164,9 +196,10 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
   bool rok[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
-    const int blk = r < MB ? r : wn * XB + (r - MB);
+    // shared slot x = r - MB holds block xlo + ((x - xlo) mod XB), if < xhi
+    const int blk = r < MB ? r : xlo + ((r - MB) - xlo % XB_ALLOC + XB_ALLOC) % XB_ALLOC;
     const int64_t i = i0 + am0 + (blk < MB ? blk : MB - 1) * 16;
-    rok[r] = i < M && blk < MB;
+    rok[r] = i < M && (r < MB || blk < xhi);
     rofs[r] = rok[r] ? p.rsC[i] : Idx(0);
   }
   Idx cofs[NC][4];
@@ -258,7 +291,8 @@ struct RealCfg {
 template <typename Idx, typename F>
 auto with_real_config(int32_t kernel, F&& f) {
   switch (kernel) {
-    case K_F32_128: return f(RealCfg<float, Idx, 128, 128, 16, 2, 2>());
+    // 8 waves (64 x 32 each): 2 waves per SIMD (+30 % over 4 waves of 64 x 64).
+    case K_F32_128: return f(RealCfg<float, Idx, 128, 128, 16, 2, 4>());
     case K_F32_64: return f(RealCfg<float, Idx, 64, 64, 16, 2, 2>());
     case K_F64_64: return f(RealCfg<double, Idx, 64, 64, 16, 2, 2>());
     default: return f(RealCfg<double, Idx, 32, 32, 16, 2, 2>());

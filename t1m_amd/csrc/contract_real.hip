@@ -292,8 +292,10 @@ template <typename Idx, typename F>
 auto with_real_config(int32_t kernel, F&& f) {
   switch (kernel) {
     // 8 waves (64 x 32 each): 2 waves per SIMD (+30 % over 4 waves of 64 x 64).
+    // (256 x 128 on 4 x 2 waves: -6 %.)
     case K_F32_128: return f(RealCfg<float, Idx, 128, 128, 16, 2, 4>());
     case K_F32_64: return f(RealCfg<float, Idx, 64, 64, 16, 2, 2>());
+    // (4 x 2 waves: -2..+7 %; 128 x 128 on 2 x 4: -10..-25 %.)
     case K_F64_64: return f(RealCfg<double, Idx, 64, 64, 16, 2, 2>());
     default: return f(RealCfg<double, Idx, 32, 32, 16, 2, 2>());
   }

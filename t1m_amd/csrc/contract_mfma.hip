@@ -26,7 +26,9 @@
 //   * scatter epilogue: lane l holds re (even l) or im (odd l) of
 //     C[i][j]; C is read-modified-written through rsC/csC (C += alpha*A.B,
 //     t1m.h:543-548), so arbitrarily strided C works and rows with unit stride
-//     give contiguous 64-byte runs per 16 lanes.
+//     give contiguous 64-byte runs per 16 lanes.  The offsets are loaded
+//     first and C is read and written in groups of row blocks (store_tile),
+//     atomically when K is split or C's elements may share addresses.
 //
 // Wave layout: WGM x WGN waves.  A wave owns BM/WGM complex rows (MB blocks
 // of 8) and NBW = floor(BN/16 / WGN) blocks of 16 columns.  When BN/16 is
@@ -34,9 +36,9 @@
 // n-wave takes MB/WGN or one more of its row blocks, reusing the A operands
 // it reads for its full blocks (a wave-uniform branch per row block; +0.5
 // to +3.5 % over reading them again; the same change in the BF16x3 kernel,
-// as a branch or with selects, lost 8-13 %).  For 144 x 144 tiles on 2 x 4 waves
-// (2 waves per SIMD) a SIMD computes 41 MFMA tiles per k chunk against an
-// even share of 40.5.
+// as a branch or with selects, lost 8-13 %).  For 144 x 144 tiles on 2 x 4
+// waves (2 waves per SIMD) a SIMD computes 41 MFMA tiles per k chunk
+// against an even share of 40.5.
 #include "internal.h"
 #include "kernel_common.h"
 

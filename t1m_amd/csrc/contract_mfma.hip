@@ -229,9 +229,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   // The shared column block reuses the A operands of this wave's share of
   // row blocks (xlo <= grp < xhi, a wave-uniform branch) instead of reading
   // them again.
-  auto mma_grp = [&](int grp, const C2& a, const BFr& f) {
-    T re, im;
-    operands(a, re, im);
+  auto mma_sel = [&](int grp, T re, T im, const BFr& f) {
 #pragma unroll
     for (int nb = 0; nb < NBW; ++nb) {
       acc[grp][nb] = mfma16(f.b[nb].x, re, acc[grp][nb]);
@@ -245,20 +243,33 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
       }
     }
   };
+  // FP64 (A not pre-expanded): the row-parity selects of step t+1 run
+  // before the MFMAs of step t, so no MFMA waits on a just-written operand.
   auto compute_fine = [&](int buf, auto&& mid) {
     BFr fb[2];
     C2 aw[LA + 1];
+    T sre[2], sim[2];
     __builtin_amdgcn_s_setprio(1);
     load_b(buf, 0, fb[0]);
 #pragma unroll
     for (int t = 0; t < LA; ++t) aw[t] = load_a(buf, t);
+    if constexpr (!AEXP) operands(aw[0], sre[0], sim[0]);
 #pragma unroll
     for (int t = 0; t < TOT; ++t) {
       const int kc = t / NG, grp = t % NG;
       if (grp == 0 && kc + 1 < KCH) load_b(buf, kc + 1, fb[(kc + 1) & 1]);
       if (t + LA < TOT) aw[(t + LA) % (LA + 1)] = load_a(buf, t + LA);
+      if constexpr (!AEXP) {
+        if (t + 1 < TOT) operands(aw[(t + 1) % (LA + 1)], sre[(t + 1) & 1], sim[(t + 1) & 1]);
+      }
       __builtin_amdgcn_sched_barrier(0);
-      mma_grp(grp, aw[t % (LA + 1)], fb[kc & 1]);
+      if constexpr (AEXP) {
+        T re, im;
+        operands(aw[t % (LA + 1)], re, im);
+        mma_sel(grp, re, im, fb[kc & 1]);
+      } else {
+        mma_sel(grp, sre[t & 1], sim[t & 1], fb[kc & 1]);
+      }
       __builtin_amdgcn_sched_barrier(0);
       if (MIDW && t == TOT / 2 - 1) {
         __builtin_amdgcn_s_setprio(0);

@@ -307,6 +307,10 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   // block.  Three products per tile: lo*hi and hi*lo first, then hi*hi.
   auto compute = [&](int buf) {
     static_assert(BK == 16, "one MFMA k-chunk per stage");
+    // Raised wave priority over the MFMA stream: +4..7 % on 64 x 64 tiles,
+    // -7..-15 % on 144 x 144 (measured on four 48^4 layouts).
+    constexpr bool PRIO = BM < 144;
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
     u32x4 bh[NBW], bl[NBW];
 #pragma unroll
     for (int nb = 0; nb < NBW; ++nb) {
@@ -336,6 +340,7 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
         accx[x] = mfma_bf16(xbh, ah, accx[x]);
       }
     }
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
   };
 
   const int nst = (K - kbeg + BK - 1) / BK;

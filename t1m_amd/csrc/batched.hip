@@ -177,6 +177,7 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
         gz.B = static_cast<const char*>(g.B) + plan->hboff[size_t(plan->L + z)];
         gz.C = static_cast<char*>(g.C) + plan->hboff[size_t(2 * plan->L + z)];
       }
+      if (plan->swap_ab) transpose_problem(gz);   // as the plan's kernel choice assumes
       bt->plans.push_back(plan);
       bt->problems.push_back(gz);
     }

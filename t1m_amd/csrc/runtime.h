@@ -53,6 +53,7 @@ struct Plan {
   std::mutex tune_mu;
   int64_t c_lo = 0, c_hi = 0;                    // C's element span
   bool c_overlap = false;                        // two index tuples of C may share an address
+  bool swap_ab = false;                          // run as C^T = B^T A^T (thin m, see get_plan)
   int64_t M = 0, N = 0, K = 0, L = 1;
   int a_mfast = 1, b_kfast = 1, idx32 = 1;
   void* maps = nullptr;
@@ -86,6 +87,7 @@ int parse_problem(const t1m_tensor* a, const char* la, const t1m_tensor* b,
                   Problem& P);
 int choose_kernel(const Problem& P, int precision);
 bool may_alias(const std::vector<int64_t>& dims, const std::vector<int64_t>& strides);
+void transpose_problem(GemmProblem& g);
 int get_plan(const Problem& P, int precision, hipStream_t stream,
              std::shared_ptr<Plan>& out);
 int64_t plan_cache_size();

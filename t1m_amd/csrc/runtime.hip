@@ -375,7 +375,8 @@ int autotune_plan(Plan& pl, const GemmProblem& g0, hipStream_t stream) {
   float best_ms = 1e30f;
   for (int k : cands) {
     // thin tiles only where C is narrow (elsewhere they cannot win)
-    if ((k == K_C64_F32_THIN || k == K_C128_F64_THIN) && pl.N > 64) continue;
+    if ((k == K_C64_F32_THIN || k == K_C128_F64_THIN) && (pl.swap_ab ? pl.M : pl.N) > 64)
+      continue;
     hipError_t e = launch_contract(g, k, stream);          // warm-up
     float kbest = 1e30f;
     for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
@@ -535,6 +536,23 @@ void fill_map(MapDesc& d, const std::vector<Mode>& modes, int which, bool out32,
 
 }  // namespace
 
+// C += A.B as C^T += B^T.A^T: operands (data, maps, conjugation, batch
+// offsets, gather directions) exchanged, C's row and column maps swapped.
+void transpose_problem(GemmProblem& g) {
+  std::swap(g.A, g.B);
+  const void* rsA = g.rsA;
+  const void* csA = g.csA;
+  g.rsA = g.csB; g.csA = g.rsB;          // rows of B^T are B's columns
+  g.rsB = csA; g.csB = rsA;              // columns of A^T are A's rows
+  std::swap(g.rsC, g.csC);
+  std::swap(g.M, g.N);
+  std::swap(g.conj_a, g.conj_b);
+  const int32_t amf = g.a_mfast;
+  g.a_mfast = g.b_kfast ? 0 : 1;         // B gathered along n <=> B^T along its rows
+  g.b_kfast = amf ? 0 : 1;               // A gathered along k <=> A^T along k
+  std::swap(g.boffA, g.boffB);
+}
+
 // True unless the layout provably gives every index tuple its own element:
 // with the modes of extent > 1 sorted by |stride|, each stride must pass the
 // reach of all smaller ones (a zero stride, or strides that interleave,
@@ -574,7 +592,18 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
   pl.M = P.M; pl.N = P.N; pl.K = P.K;
   pl.dtype = P.dtype;
   pl.precision = precision;
-  pl.kernel = choose_kernel(P, precision);
+  // A complex contraction with at most 16 free rows of A (and a wider C)
+  // runs transposed, C^T = B^T . A^T (complex products commute), so the
+  // thin 128 x 16 tiles cover it: every kernel sees M and N swapped, A and
+  // B (maps, data, conjugation, gather directions) exchanged.
+  const bool cplx_native = (P.dtype == T1M_COMPLEX64 || P.dtype == T1M_COMPLEX128) &&
+                           precision == T1M_PRECISION_NATIVE;
+  pl.swap_ab = cplx_native && P.M <= 16 && P.N > 16;
+  {
+    Problem Q = P;
+    if (pl.swap_ab) std::swap(Q.M, Q.N);
+    pl.kernel = choose_kernel(Q, precision);
+  }
   for (size_t i = 0; i < P.dc.size(); ++i) {
     const int64_t span = (P.dc[i] > 0 ? P.dc[i] - 1 : 0) * P.sc[i];
     if (span < 0) pl.c_lo += span; else pl.c_hi += span;
@@ -892,6 +921,7 @@ int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
     g.boffB = plan->boff + plan->L;
     g.boffC = plan->boff + 2 * plan->L;
   }
+  if (plan->swap_ab) transpose_problem(g);
   if (opt.autotune && !plan->tuned.load() && (rc = autotune_plan(*plan, g, stream))) return rc;
   hipError_t e = launch_contract(g, plan->kernel, stream);
   if (e != hipSuccess)

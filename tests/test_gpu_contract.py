@@ -857,3 +857,33 @@ def test_static_tile_rule():
                     precision="bf16x3") == "c64_bf16x3_144x144"
         assert kern("ac", "cb", "ab", dict(a=2568, b=2560, c=64)) == "c64_f32mfma_64x64"
         assert kern("ea", "ebcd", "abcd", dict(a=36, b=36, c=36, d=36, e=36)) == "c64_f32mfma_64x64"
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_thin_m_runs_transposed(dtype):
+    """A with at most 16 free rows (and a wider C) runs as C^T = B^T A^T on
+    every kernel, with conjugation, complex alpha, a batch label and host
+    pointers; the static rule picks the thin tiles for it."""
+    rng = np.random.default_rng(31)
+    kernels = ALL_KERNELS[dtype][:-1] if dtype == np.complex64 else ALL_KERNELS[dtype]
+    for la, lb, lc, ext in (("ea", "ebcd", "abcd", dict(a=12, b=9, c=10, d=7, e=20)),
+                            ("zea", "zeb", "zab", dict(z=3, a=5, b=70, e=33)),
+                            ("ae", "ebc", "cba", dict(a=16, b=40, c=3, e=8))):
+        A = rand(rng, [ext[x] for x in la], dtype)
+        B = rand(rng, [ext[x] for x in lb], dtype)
+        C0 = rand(rng, [ext[x] for x in lc], dtype)
+        alpha = complex(0.5, -0.75)
+        want = oracle_einsum(A, la, B, lb, C0, lc, alpha=alpha, conj_a=True)
+        for kern, prec, tolkey in kernels:
+            with forced(kern):
+                got = run(A, la, B, lb, C0, lc, precision=prec, alpha=alpha, conj_a=True)
+            err = rel_err(got - C0, want - C0)
+            print(f"  {la},{lb}->{lc} {ext} {kern} rel={err:.2e}")
+            assert err < TOL[tolkey], (la, lb, lc, kern, err)
+    thin = "c64_f32mfma_128x16" if dtype == np.complex64 else "c128_f64mfma_128x16"
+    tdt = torch.complex64 if dtype == np.complex64 else torch.complex128
+    mk = lambda lab, ext: torch.zeros([ext[x] for x in lab], dtype=tdt, device="cuda")
+    ext = dict(a=12, b=36, c=36, d=36, e=36)
+    with forced(None):
+        assert t1m_amd.plan_kernel(mk("ea", ext), "ea", mk("ebcd", ext), "ebcd",
+                                   mk("abcd", ext), "abcd") == thin

@@ -334,6 +334,10 @@ std::string tune_key(const Plan& pl) {
     const size_t n = strlen(buf);
     snprintf(buf + n, sizeof(buf) - n, "|L%lld", (long long)pl.L);
   }
+  if (pl.swap_ab) {
+    const size_t n = strlen(buf);
+    snprintf(buf + n, sizeof(buf) - n, "|T");
+  }
   return buf;
 }
 
@@ -592,13 +596,18 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
   pl.M = P.M; pl.N = P.N; pl.K = P.K;
   pl.dtype = P.dtype;
   pl.precision = precision;
-  // A complex contraction with at most 16 free rows of A (and a wider C)
-  // runs transposed, C^T = B^T . A^T (complex products commute), so the
-  // thin 128 x 16 tiles cover it: every kernel sees M and N swapped, A and
-  // B (maps, data, conjugation, gather directions) exchanged.
+  // Transposed plans, C^T = B^T . A^T (products commute): every kernel sees
+  // M and N swapped, A and B (maps, data, conjugation, gather directions)
+  // exchanged.  (1) A complex contraction with at most 16 free rows of A
+  // and a wider C, so the thin 128 x 16 tiles cover it; (2) C whose unit
+  // stride lies on a free mode of B (n-major), so the epilogue's 16-lane
+  // runs (consecutive rows) become contiguous -- unless C is thin in n.
   const bool cplx_native = (P.dtype == T1M_COMPLEX64 || P.dtype == T1M_COMPLEX128) &&
                            precision == T1M_PRECISION_NATIVE;
-  pl.swap_ab = cplx_native && P.M <= 16 && P.N > 16;
+  const bool thin_m = cplx_native && P.M <= 16 && P.N > 16;
+  const bool n_major = precision != T1M_PRECISION_SIMPLE && min_stride(P.n, 2) < min_stride(P.m, 2) &&
+                       !(cplx_native && P.N <= 16) && P.M > 16;
+  pl.swap_ab = thin_m || n_major;
   {
     Problem Q = P;
     if (pl.swap_ab) std::swap(Q.M, Q.N);

@@ -254,10 +254,12 @@ int choose_kernel(const Problem& P, int precision) {
     case T1M_COMPLEX128:
       if (P.N <= 16) return K_C128_F64_THIN;
       return tiles(64, 64) >= cu_count() ? K_C128_F64_64 : K_C128_F64_32;
-    case T1M_FLOAT32:
+    case T1M_FLOAT32:   // narrow C (N <= 32): the small tiles waste less
+      if (P.N <= 32) return K_F32_64;
       return tiles(128, 128) >= cu_count() ? K_F32_128 : K_F32_64;
     case T1M_FLOAT64:
-      return tiles(64, 64) >= 2.0 * cu_count() ? K_F64_64 : K_F64_32;
+      if (P.N <= 32) return K_F64_32;
+      return tiles(64, 64) >= cu_count() ? K_F64_64 : K_F64_32;
     default:
       return K_SIMPLE;
   }

@@ -757,6 +757,20 @@ ALL_KERNELS = {np.complex64: KERNELS_C64, np.complex128: KERNELS_C128,
                np.float32: REAL_BATCH[np.float32], np.float64: REAL_BATCH[np.float64]}
 
 
+def _scaled_err(got, want, A, la, B, lb, C0, lc, ext, alpha=1.0):
+    """Error of C0 + alpha A.B against the oracle, relative to the scale of
+    what was summed: |alpha| |A|.|B| (the contraction of the magnitudes)
+    plus |C0|.  Small sums with cancellation (dot products, contractions to
+    a scalar) are so held to the precision of their terms, not of their
+    possibly tiny result."""
+    absa = np.asarray(np.abs(A), dtype=np.float64)     # stays an array for 0-d tensors
+    absb = np.asarray(np.abs(B), dtype=np.float64)
+    mag = oracle_einsum(absa, la, absb, lb, np.zeros([ext[x] for x in lc], np.float64), lc)
+    d = np.asarray(got, np.complex128).ravel() - np.asarray(want, np.complex128).ravel()
+    scale = abs(alpha) * np.linalg.norm(mag.ravel()) + np.linalg.norm(np.abs(C0).ravel())
+    return np.linalg.norm(d) / max(scale, 1e-300)
+
+
 def _random_problem(rng):
     """Random labels (contracted, free A, free B, sometimes a batch label)
     with extents 1..9."""
@@ -811,8 +825,8 @@ def test_fuzz_random_contractions(seed):
                 t1m_amd.contract(a, la, b, lb, c, lc, precision=prec, alpha=alpha,
                                  conj_a=conj_a, conj_b=conj_b)
             got = c.cpu().numpy()
-            err = rel_err(got - C0, want - C0)
-            print(f"  {la},{lb}->{lc} {ext} {np.dtype(dtype).name} {kern} rel={err:.2e}")
+            err = _scaled_err(got, want, A, la, B, lb, C0, lc, ext, alpha)
+            print(f"  {la},{lb}->{lc} {ext} {np.dtype(dtype).name} {kern} err={err:.2e}")
             assert err < TOL[tolkey], (la, lb, lc, ext, kern, err)
 
 
@@ -835,8 +849,8 @@ def test_fuzz_blocked_orders(seed):
             a, b, c = _random_layout(rng, A), _random_layout(rng, B), _random_layout(rng, C0)
             with forced(kern):
                 t1m_amd.contract(a, la, b, lb, c, lc, precision=prec)
-            err = rel_err(c.cpu().numpy() - C0, want - C0)
-            print(f"  {la},{lb}->{lc} {ext} {np.dtype(dtype).name} {kern} rel={err:.2e}")
+            err = _scaled_err(c.cpu().numpy(), want, A, la, B, lb, C0, lc, ext)
+            print(f"  {la},{lb}->{lc} {ext} {np.dtype(dtype).name} {kern} err={err:.2e}")
             assert err < TOL[tolkey], (la, lb, lc, ext, kern, err)
 
 

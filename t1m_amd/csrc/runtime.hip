@@ -310,9 +310,10 @@ void load_tune_cache(TuneCache& T) {
 }
 
 void append_tune_cache(TuneCache& T, const std::string& key, int kernel) {
-  const std::string dir = T.path.substr(0, T.path.find_last_of('/'));
-  std::string cmd;
-  // Create the directory without a shell: mkdir each prefix.
+  // Create the file's directory without a shell: mkdir each prefix (a bare
+  // file name lives in the working directory).
+  const size_t slash = T.path.find_last_of('/');
+  const std::string dir = slash == std::string::npos ? std::string() : T.path.substr(0, slash);
   for (size_t i = 1; i <= dir.size(); ++i)
     if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
   FILE* f = fopen(T.path.c_str(), "a");
@@ -374,9 +375,14 @@ int autotune_plan(Plan& pl, const GemmProblem& g0, hipStream_t stream) {
   }
   GemmProblem g = g0;
   g.C = scratch - pl.c_lo * es;
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    const hipError_t err = hipGetLastError();
+    if (e0) (void)hipEventDestroy(e0);
+    (void)hipFree(scratch);
+    return set_error(T1M_ERROR_DEVICE, "hipEventCreate for autotuning failed: %s",
+                     hipGetErrorString(err));
+  }
   int best = pl.kernel;
   float best_ms = 1e30f;
   for (int k : cands) {

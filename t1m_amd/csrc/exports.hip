@@ -16,6 +16,18 @@
 
 using namespace t1m_hip;
 
+namespace {
+
+// A view's counts and arrays: non-negative counts, and every array the
+// kernels index present when the view is not empty.
+bool view_ok(const t1m_view& v) {
+  if (v.nrs < 0 || v.ncs < 0 || v.nrbs < 0 || v.ncbs < 0) return false;
+  if (v.nrs == 0 || v.ncs == 0) return true;
+  return v.rs && v.cs && v.rbs && v.cbs && v.nrbs > 0 && v.ncbs > 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int t1m_index_bundles(const char* la, const char* lb, const char* lc,
@@ -87,6 +99,8 @@ int t1m_index_maps(const t1m_tensor* t, const int64_t* modes, int32_t nmodes,
     const int64_t x = modes[i];
     if (x < 0 || x >= t->rank)
       return set_error(T1M_ERROR_INVALID_ARGUMENT, "mode %lld out of range", (long long)x);
+    if (t->dims[x] < 0)
+      return set_error(T1M_ERROR_INVALID_ARGUMENT, "negative extent %lld", (long long)t->dims[x]);
     d.dims[i] = t->dims[x];
     if (t->strides) {
       d.strides[i] = t->strides[x];
@@ -115,6 +129,8 @@ int t1m_pack_block(int32_t dtype, int32_t label, const t1m_view* view,
   const bool cplx = dtype == T1M_COMPLEX64 || dtype == T1M_COMPLEX128;
   if (view->br <= 0 || view->bc <= 0 || (cplx && (view->br < 2 || (label == 0 && view->bc < 2))))
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "block sizes too small for the panel format");
+  if (!view_ok(*view) || length < 0)
+    return set_error(T1M_ERROR_INVALID_ARGUMENT, "bad view counts or arrays");
   HIP_TRY(launch_pack_block(dtype, label, *view, length, src, dest,
                             static_cast<hipStream_t>(stream)));
   return T1M_SUCCESS;
@@ -127,6 +143,8 @@ int t1m_unpack(int32_t dtype, const t1m_view* view, const void* src,
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "null argument");
   if (dtype_size(dtype) == 0)
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+  if (!view_ok(*view) || view->br <= 0)
+    return set_error(T1M_ERROR_INVALID_ARGUMENT, "bad view counts, arrays or block size");
   HIP_TRY(launch_unpack(dtype, *view, src, dest, static_cast<hipStream_t>(stream)));
   return T1M_SUCCESS;
 }

@@ -103,3 +103,23 @@ def test_bad_options_and_null_arguments():
     assert lib.t1m_index_bundles(None, b"", b"", None, None) == 1
     assert lib.t1m_batch_destroy(None) == 0
     assert lib.t1m_batch_launch(None, None) == 1
+
+
+def test_view_and_extent_validation():
+    # Rejected on the host before any device work.
+    lib = t1m_amd.load()
+    one = ctypes.c_void_p(16)          # never dereferenced: validation fails first
+    bad_counts = t1m_amd.View(4, 4, one, -1, one, 2, one, 1, one, 1)
+    assert lib.t1m_pack_block(t1m_amd.FLOAT32, 0, ctypes.byref(bad_counts), 8, one, one, None) == 1
+    assert b"view" in lib.t1m_last_error()
+    missing = t1m_amd.View(4, 4, one, 3, one, 2, None, 1, one, 1)
+    assert lib.t1m_unpack(t1m_amd.FLOAT32, ctypes.byref(missing), one, one, None) == 1
+    no_blocks = t1m_amd.View(4, 4, one, 3, one, 2, one, 0, one, 1)
+    assert lib.t1m_pack_block(t1m_amd.FLOAT32, 1, ctypes.byref(no_blocks), 8, one, one, None) == 1
+    bad_ld = t1m_amd.View(0, 4, one, 3, one, 2, one, 1, one, 1)
+    assert lib.t1m_unpack(t1m_amd.FLOAT32, ctypes.byref(bad_ld), one, one, None) == 1
+    dims = (ctypes.c_int64 * 2)(3, -2)
+    t = t1m_amd.TensorDesc(t1m_amd.FLOAT32, 2, dims, None, None)
+    modes = (ctypes.c_int64 * 2)(0, 1)
+    assert lib.t1m_index_maps(ctypes.byref(t), modes, 2, 4, one, None, None) == 1
+    assert b"negative extent" in lib.t1m_last_error()

@@ -75,6 +75,11 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   // Next stage's LDS writes in the middle of the stage; 64-bit offsets on
   // the 144 x 144 tiles leave no registers for them.
   constexpr bool MIDW = sizeof(Idx) == 4 || BM * BN < 144 * 144;
+  // Global loads two stages ahead (a second set of data registers) on the
+  // 144 x 144 FP32 tiles with 32-bit offsets: +1.2..3.3 % at 2304^3 over
+  // the four gather-direction pairs.  On the 64 x 64 tiles the extra
+  // registers cost a workgroup per CU (-6 %), on FP64 they measured -1..+1 %.
+  constexpr bool DEEP = AEXP && !PREC && BM == 144 && sizeof(Idx) == 4;
   // LDS layouts.  A holds each complex element twice, pre-expanded for the
   // two row parities of 1M: (re, -im) for even lanes, (im, re) for odd ones,
   // so the MFMA operands come straight from the LDS reads.  Both tiles are
@@ -134,15 +139,17 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   const T sa = p.conj_a ? T(-1) : T(1);
   const T sb = p.conj_b ? T(-1) : T(1);
 
-  auto swrite = [&](int buf, int p0) {
-    ga.finish(utid, p0, K, sa);
-    gb.finish(utid, p0, K, sb);
+  // Data registers of a stage: one set, or two when the loads run two
+  // stages ahead (DEEP).
+  using VA = C2[GA::E];
+  using VB = C2[GB::E];
+  auto swrite = [&](int buf, int p0, const VA& va, const VB& vb) {
 #pragma unroll
     for (int u = 0; u < GA::E; ++u) {
       const unsigned e = GA::elem(utid, u);
       const int k = GA::kc(e);
       if (GA::valid(utid, u)) {
-        const C2 v = ga.val[u];
+        const C2 v = ga.finished(utid, p0, K, sa, u, va[u]);
         if constexpr (AEXP) {
           AExp w;
           w.e.x = v.x; w.e.y = -v.y;
@@ -157,7 +164,8 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
     for (int u = 0; u < GB::E; ++u) {
       const unsigned e = GB::elem(utid, u);
       const int k = GB::kc(e);
-      if (GB::valid(utid, u)) sB[buf][k][GB::xc(e) ^ swb(k)] = gb.val[u];
+      if (GB::valid(utid, u))
+        sB[buf][k][GB::xc(e) ^ swb(k)] = gb.finished(utid, p0, K, sb, u, vb[u]);
     }
   };
 
@@ -358,24 +366,57 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   gb.load_data(B);
   ga.load_k(utid, kbeg + BK, K, p.csA);
   gb.load_k(utid, kbeg + BK, K, p.rsB);
-  if constexpr (PREC) {
-    if (!atomic) load_c_tile<T, NR, NC>(has, addr, cv);
-  }
-  swrite(0, kbeg);
-  __syncthreads();
-  // The loop body is branch-free (a conditional prefetch made the compiler
-  // wait for every load in flight at the top of each iteration); the last
-  // stage is peeled off so no data is fetched past the end.  The k-offset
-  // prefetch of the stage after the last reads clamped, valid indices.
-  for (int s = 0; s + 1 < nst; ++s) {
-    ga.load_data(A);
-    gb.load_data(B);
-    ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
-    gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
-    compute(s & 1, [&] { swrite((s + 1) & 1, kbeg + (s + 1) * BK); });
+  if constexpr (DEEP) {
+    // Two stages of data in flight: iteration s loads stage s + 2 into the
+    // set that stage s left, while the other set (stage s + 1) is written
+    // to LDS in the middle of the stage.  Loads past the last stage read
+    // clamped, valid k offsets and are never written.
+    VA va1;
+    VB vb1;
+    ga.load_data_to(A, va1);
+    gb.load_data_to(B, vb1);
+    ga.load_k(utid, kbeg + 2 * BK, K, p.csA);
+    gb.load_k(utid, kbeg + 2 * BK, K, p.rsB);
+    swrite(0, kbeg, ga.val, gb.val);
     __syncthreads();
+    auto body = [&](int s, VA& la, VB& lb, const VA& wa, const VB& wb) {
+      ga.load_data_to(A, la);
+      gb.load_data_to(B, lb);
+      ga.load_k(utid, kbeg + (s + 3) * BK, K, p.csA);
+      gb.load_k(utid, kbeg + (s + 3) * BK, K, p.rsB);
+      compute(s & 1, [&] { swrite((s + 1) & 1, kbeg + (s + 1) * BK, wa, wb); });
+      __syncthreads();
+    };
+    int s = 0;
+    for (; s + 2 < nst; s += 2) {
+      body(s, ga.val, gb.val, va1, vb1);
+      body(s + 1, va1, vb1, ga.val, gb.val);
+    }
+    if (s + 1 < nst) {
+      body(s, ga.val, gb.val, va1, vb1);
+      ++s;
+    }
+    compute(s & 1, [] {});
+  } else {
+    if constexpr (PREC) {
+      if (!atomic) load_c_tile<T, NR, NC>(has, addr, cv);
+    }
+    swrite(0, kbeg, ga.val, gb.val);
+    __syncthreads();
+    // The loop body is branch-free (a conditional prefetch made the compiler
+    // wait for every load in flight at the top of each iteration); the last
+    // stage is peeled off so no data is fetched past the end.  The k-offset
+    // prefetch of the stage after the last reads clamped, valid indices.
+    for (int s = 0; s + 1 < nst; ++s) {
+      ga.load_data(A);
+      gb.load_data(B);
+      ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
+      gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
+      compute(s & 1, [&] { swrite((s + 1) & 1, kbeg + (s + 1) * BK, ga.val, gb.val); });
+      __syncthreads();
+    }
+    compute((nst - 1) & 1, [] {});
   }
-  compute((nst - 1) & 1, [] {});
 
   if constexpr (PREC) {
     if (atomic)

@@ -224,9 +224,20 @@ struct TileGather {
       koff[u] = kmap[kg < K ? kg : K - 1];
     }
   }
-  __device__ void load_data(const T2* __restrict__ base) {
+  __device__ void load_data(const T2* __restrict__ base) { load_data_to(base, val); }
+  __device__ void load_data_to(const T2* __restrict__ base, T2 (&out)[E]) const {
 #pragma unroll
-    for (int u = 0; u < E; ++u) val[u] = base[xoff[u] + koff[u]];
+    for (int u = 0; u < E; ++u) out[u] = base[xoff[u] + koff[u]];
+  }
+  // Entry u of a stage starting at p0: zero when out of range, conjugated
+  // if asked (sign = -1).
+  template <typename T>
+  __device__ T2 finished(unsigned tid, int p0, int K, T sign, int u, T2 v) const {
+    const int kg = p0 + kc(elem(tid, u));
+    const bool ok = ((xok >> u) & 1u) && kg < K;
+    v.x = ok ? v.x : T(0);
+    v.y = ok ? sign * v.y : T(0);
+    return v;
   }
   // Zero out-of-range entries (real data).
   __device__ void finish_real(unsigned tid, int p0, int K) {

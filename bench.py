@@ -496,12 +496,81 @@ def c5_cpu_baseline(dims, seconds_hint=15.0):
                        f"asb,btc->astc ({fl:.3e} flop, {spent:.2f} s, 1 thread)")}
 
 
+def bench_maps(args, torch, t1m):
+    """HBM rates of the device index-map and panel kernels at bandwidth-sized
+    inputs (algorithmic bytes per launch / mean launch time, HIP events on
+    the launch stream):
+      k_scatter_maps  scatter vector of all modes of a 64^4 tensor (8 B written per entry)
+      k_block_scatter its block-scatter vector, block 6 (8 B read per entry)
+      k_pack_block    1M A panels of the 48^4 'abcd' operand, rows (a,c) x cols (b,d),
+                      MR = 6, KP = 4 (8 B read + 16 B written per complex element)
+      k_unpack        the 48^4 'acef' C tile, rows (a,c) x cols (e,f), leading dimension
+                      2 x 2304 (8 B tile read + 8 B C read + 8 B C written per element)"""
+    import ctypes
+    L = t1m.load()
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    out = {}
+
+    def rate(name, fn, nbytes):
+        _, kern = timed_region(torch, stream, fn, args.steps, args.warmup)
+        ms = sum(kern) / len(kern)
+        out[name] = {"GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1), "us": round(ms * 1e3, 2),
+                     "bytes": int(nbytes), "frac_of_hbm_peak": round(nbytes / (ms * 1e-3) / 8e12, 4)}
+
+    def desc(dims):
+        d = (ctypes.c_int64 * len(dims))(*dims)
+        return t1m.TensorDesc(t1m.COMPLEX64, len(dims), d, None, None), d
+
+    # scatter / block scatter over 64^4
+    big, keep = desc([64] * 4)
+    n = 64 ** 4
+    modes = (ctypes.c_int64 * 4)(0, 1, 2, 3)
+    scat = torch.empty(n, dtype=torch.int64, device="cuda")
+    bs = torch.empty(-(-n // 6), dtype=torch.int64, device="cuda")
+    rate("k_scatter_maps", lambda: L.t1m_index_maps(ctypes.byref(big), modes, 4, 0,
+                                                    scat.data_ptr(), None, sh), 8 * n)
+    rate("k_scatter_maps+k_block_scatter",
+         lambda: L.t1m_index_maps(ctypes.byref(big), modes, 4, 6, scat.data_ptr(),
+                                  bs.data_ptr(), sh), 8 * n + 8 * n + 8 * bs.numel())
+
+    # 1M A panels of the 48^4 operand
+    dims = [48] * 4
+    rows, cols = (0, 2), (1, 3)
+    br, bc = 6, 4
+    rs, rbs = t1m.index_maps(dims, None, rows, br)
+    cs, cbs = t1m.index_maps(dims, None, cols, bc)
+    m = k = 48 * 48
+    src = torch.complex(torch.rand(48 ** 4, device="cuda"), torch.rand(48 ** 4, device="cuda"))
+    length = 2 * k
+    dest = torch.zeros(-(-2 * m // br) * br * length + 64, dtype=torch.float32, device="cuda")
+    v = t1m._view(br, bc, rs, cs, rbs, cbs)
+    rate("k_pack_block (1M A, complex64)",
+         lambda: L.t1m_pack_block(t1m.COMPLEX64, 0, ctypes.byref(v), length, src.data_ptr(),
+                                  dest.data_ptr(), sh), 24 * m * k)
+
+    # unpack a 2304 x 2304 complex tile into the 48^4 C
+    rs2, rbs2 = t1m.index_maps(dims, None, (0, 1), m)
+    cs2, cbs2 = t1m.index_maps(dims, None, (2, 3), k)
+    tile = torch.rand(2 * m * k, device="cuda")
+    cdst = torch.zeros(48 ** 4, dtype=torch.complex64, device="cuda")
+    v2 = t1m._view(2 * m, k, rs2, cs2, rbs2, cbs2)
+    rate("k_unpack (complex64)",
+         lambda: L.t1m_unpack(t1m.COMPLEX64, ctypes.byref(v2), tile.data_ptr(), cdst.data_ptr(),
+                              sh), 24 * m * k)
+    torch.cuda.synchronize()
+    print(json.dumps({"metric": "index-map / panel kernels, HBM GB/s (algorithmic bytes)",
+                      "unit": "GB/s", "peak": 8000, "steps": args.steps, "warmup": args.warmup,
+                      "kernels": out}), flush=True)
+    del keep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c5"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c5", "maps"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-errors", action="store_true", help="skip the error report")
     ap.add_argument("--no-library", action="store_true",
@@ -529,6 +598,9 @@ def main():
 
     if args.config == "c5":
         bench_c5(args, torch, t1m)
+        return
+    if args.config == "maps":
+        bench_maps(args, torch, t1m)
         return
     cfg = CONFIGS[args.config]
     fl, (m, n, k) = flops(cfg)

@@ -19,18 +19,29 @@ namespace t1m_hip {
 
 namespace {
 
+template <typename U>
+__device__ inline uint64_t scatter_entry(const MapDesc& d, int64_t e) {
+  U r = U(e);
+  uint64_t off = 0;
+  for (int j = 0; j < d.nmodes; ++j) {
+    const U ext = U(d.dims[j]);
+    const U q = r / ext;
+    off += uint64_t(r - q * ext) * uint64_t(d.strides[j]);
+    r = q;
+  }
+  return off;
+}
+
 __global__ void k_scatter_maps(MapBatch batch) {
   const MapDesc& d = batch.maps[blockIdx.y];
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  // A bundle of fewer than 2^32 entries has every extent below 2^32 too:
+  // its digits come from 32-bit divisions (64-bit ones cost ~4x the
+  // instructions and made the kernel issue-bound at a quarter of HBM speed).
+  const bool narrow = uint64_t(d.len) <= 0xffffffffull;
   for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < d.len;
        e += stride) {
-    uint64_t r = uint64_t(e), off = 0;
-    for (int j = 0; j < d.nmodes; ++j) {
-      const uint64_t ext = uint64_t(d.dims[j]);
-      const uint64_t digit = r % ext;
-      r /= ext;
-      off += digit * uint64_t(d.strides[j]);
-    }
+    const uint64_t off = narrow ? scatter_entry<uint32_t>(d, e) : scatter_entry<uint64_t>(d, e);
     if (d.out32)
       static_cast<int32_t*>(d.out)[e] = int32_t(int64_t(off));
     else

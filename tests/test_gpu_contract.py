@@ -234,6 +234,26 @@ def test_ragged_all_kernels(case, idx64):
             assert err < TOL[tolkey], (kern, err)
 
 
+@pytest.mark.parametrize("layout", [("ik", "kj"), ("ki", "kj"), ("ik", "jk"), ("ki", "jk")])
+def test_144_tiles_every_stage_count(layout):
+    """The 144 x 144 FP32 tiles keep two stages of global loads in flight
+    (pairs of stages per loop trip, one stage left over or not): K of 1 to 6
+    stages of 16, ragged or whole, for each pair of gather directions."""
+    la, lb = layout
+    rng = np.random.default_rng(11)
+    for k in (1, 15, 16, 17, 32, 47, 48, 64, 80, 96):
+        ext = {"i": 150, "j": 161, "k": k}
+        A = rand(rng, [ext[x] for x in la], np.complex64)
+        B = rand(rng, [ext[x] for x in lb], np.complex64)
+        C0 = rand(rng, [ext[x] for x in "ij"], np.complex64)
+        want = oracle_einsum(A, la, B, lb, C0, "ij")
+        with forced("c64_f32mfma_144x144"):
+            got = run(A, la, B, lb, C0, "ij")
+        err = rel_err(got - C0, want - C0)
+        print(f"  {la},{lb}->ij k={k} rel={err:.2e}")
+        assert err < TOL["fp32"], (k, err)
+
+
 REAL_KERNELS = {np.float32: ["f32mfma_128x128", "f32mfma_64x64", "simple"],
                 np.float64: ["f64mfma_64x64", "f64mfma_32x32", "simple"]}
 

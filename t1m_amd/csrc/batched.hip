@@ -57,7 +57,7 @@ int cu_count_batch() {
 // Tile family for a whole batch: the small tiles, which keep the many small
 // instances of a sweep busy (one instance rarely fills the 144-wide tiles).
 int batch_kernel(int dtype, int precision) {
-  if (const char* f = getenv("T1M_FORCE_KERNEL")) {
+  if (const char* f = env_override("T1M_FORCE_KERNEL")) {
     const int k = atoi(f);
     // batches run the small square tiles only
     if (kernel_fits(k, dtype, precision) && k != K_C64_F32_144 && k != K_C64_BF16X3_144 &&

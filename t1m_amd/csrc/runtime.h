@@ -87,6 +87,9 @@ int parse_problem(const t1m_tensor* a, const char* la, const t1m_tensor* b,
                   Problem& P);
 int choose_kernel(const Problem& P, int precision);
 bool may_alias(const std::vector<int64_t>& dims, const std::vector<int64_t>& strides);
+// A test / tuning override from the environment (T1M_FORCE_KERNEL,
+// T1M_FORCE_IDX64); unset and empty read alike (nullptr).
+const char* env_override(const char* name);
 void transpose_problem(GemmProblem& g);
 int get_plan(const Problem& P, int precision, hipStream_t stream,
              std::shared_ptr<Plan>& out);

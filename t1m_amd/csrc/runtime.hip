@@ -227,7 +227,7 @@ bool kernel_fits(int kernel, int dtype, int precision) {
 // Calibrated on the 48^4, 256x4x256 and reference-suite contractions.
 int choose_kernel(const Problem& P, int precision) {
   if (P.K >= (int64_t(1) << 31)) return K_SIMPLE;     // MFMA kernels use int k
-  if (const char* f = getenv("T1M_FORCE_KERNEL")) {   // tests / tuning; ignored
+  if (const char* f = env_override("T1M_FORCE_KERNEL")) {   // tests / tuning; ignored
     const int k = atoi(f);                            // when it does not fit
     if (kernel_fits(k, P.dtype, precision)) return k;
   }
@@ -284,7 +284,7 @@ TuneCache& tune_cache() {
 }
 
 std::string tune_path() {
-  if (const char* p = getenv("T1M_TUNE_CACHE")) return p;
+  if (const char* p = getenv("T1M_TUNE_CACHE"); p && *p) return p;
   const char* home = getenv("HOME");
   return std::string(home ? home : "/tmp") + "/.cache/t1m_mi355x/tuning.txt";
 }
@@ -350,7 +350,7 @@ int autotune_plan(Plan& pl, const GemmProblem& g0, hipStream_t stream) {
   std::lock_guard<std::mutex> guard(pl.tune_mu);
   if (pl.tuned.load()) return T1M_SUCCESS;
   const std::vector<int> cands = kernel_candidates(pl.dtype, pl.precision);
-  if (cands.size() < 2 || getenv("T1M_FORCE_KERNEL")) {
+  if (cands.size() < 2 || env_override("T1M_FORCE_KERNEL")) {
     pl.tuned = true;
     return T1M_SUCCESS;
   }
@@ -442,8 +442,8 @@ std::string plan_key(const Problem& P, int precision, int device) {
   std::string k;
   char buf[64];
   // Test / tuning overrides change the plan, so they are part of its key.
-  const char* fk = getenv("T1M_FORCE_KERNEL");
-  const char* f64 = getenv("T1M_FORCE_IDX64");
+  const char* fk = env_override("T1M_FORCE_KERNEL");
+  const char* f64 = env_override("T1M_FORCE_IDX64");
   snprintf(buf, sizeof(buf), "%d|%d|%d|%s|%s|", device, P.dtype, precision,
            fk ? fk : "", f64 ? f64 : "");
   k += buf;
@@ -570,6 +570,11 @@ void transpose_problem(GemmProblem& g) {
 // reach of all smaller ones (a zero stride, or strides that interleave,
 // fail).  Outputs that may alias are then added atomically, so every
 // contribution lands as in a serial loop.
+const char* env_override(const char* name) {
+  const char* v = getenv(name);
+  return (v && *v) ? v : nullptr;
+}
+
 bool may_alias(const std::vector<int64_t>& dims, const std::vector<int64_t>& strides) {
   std::vector<std::pair<int64_t, int64_t>> m;   // (|stride|, extent)
   for (size_t i = 0; i < dims.size(); ++i)
@@ -645,7 +650,7 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
   const int64_t offB = max_abs_offset(P.k, 1) + max_abs_offset(P.n, 1);
   const int64_t offC = max_abs_offset(P.m, 2) + max_abs_offset(P.n, 2);
   pl.idx32 = (offA <= lim && offB <= lim && offC <= lim) ? 1 : 0;
-  if (const char* f = getenv("T1M_FORCE_IDX64"))
+  if (const char* f = env_override("T1M_FORCE_IDX64"))
     if (atoi(f) != 0) pl.idx32 = 0;
   const size_t isz = pl.idx32 ? 4 : 8;
 
@@ -710,7 +715,7 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
     if (eb != hipSuccess)
       return set_error(T1M_ERROR_DEVICE, "batch offsets upload failed: %s", hipGetErrorString(eb));
   }
-  if (!getenv("T1M_FORCE_KERNEL")) {     // a tuning result from an earlier run
+  if (!env_override("T1M_FORCE_KERNEL")) {     // a tuning result from an earlier run
     TuneCache& T = tune_cache();
     std::lock_guard<std::mutex> lock(T.mu);
     load_tune_cache(T);
@@ -837,7 +842,7 @@ void call_signature(const t1m_tensor* const* t, const char* const* l, int precis
   sig.push_back(dev);
   sig.push_back(precision);
   for (const char* env : {"T1M_FORCE_KERNEL", "T1M_FORCE_IDX64"}) {
-    const char* v = getenv(env);
+    const char* v = env_override(env);
     int64_t h = v ? 1469598103934665603LL : 0;
     if (v)
       for (const char* q = v; *q; ++q) h = (h ^ uint8_t(*q)) * 1099511628211LL;

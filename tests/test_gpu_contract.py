@@ -891,6 +891,16 @@ def test_static_tile_rule():
                     precision="bf16x3") == "c64_bf16x3_144x144"
         assert kern("ac", "cb", "ab", dict(a=2568, b=2560, c=64)) == "c64_f32mfma_64x64"
         assert kern("ea", "ebcd", "abcd", dict(a=36, b=36, c=36, d=36, e=36)) == "c64_f32mfma_64x64"
+    # an empty override reads as unset
+    old = os.environ.get("T1M_FORCE_KERNEL")
+    os.environ["T1M_FORCE_KERNEL"] = ""
+    try:
+        assert kern("abcd", "dbef", "acef", {ch: 48 for ch in "abcdef"}) == "c64_f32mfma_144x144"
+    finally:
+        if old is None:
+            del os.environ["T1M_FORCE_KERNEL"]
+        else:
+            os.environ["T1M_FORCE_KERNEL"] = old
 
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])

@@ -663,6 +663,63 @@ def test_batch_families(dtype, prec, tol):
     batch.close()
 
 
+def test_batched_vjp_mps_sweep():
+    """The MPS sweep's gradient as the benchmark computes it: forward batch,
+    then grad_A = grad_C . conj(B) and grad_B = conj(A) . grad_C as two more
+    persistent batches (complex<float>, L = sum |C|^2, grad_C = 2 C), checked
+    against the oracle's conjugated label loops and complex128 central
+    differences (h = 1e-6) of the loss."""
+    rng = np.random.default_rng(23)
+    la, lb, lc = "asb", "btc", "astc"
+    dims = [tuple(int(x) for x in rng.choice(np.arange(16, 81, 16), 3)) for _ in range(6)]
+    As = [rand(rng, (dl, 4, db), np.complex64) for dl, db, dr in dims]
+    Bs = [rand(rng, (db, 4, dr), np.complex64) for dl, db, dr in dims]
+    Cs = [to_dev(np.zeros((dl, 4, 4, dr), np.complex64, order="F")) for dl, db, dr in dims]
+    a_dev, b_dev = [to_dev(A) for A in As], [to_dev(B) for B in Bs]
+    fwd = t1m_amd.Batch(a_dev, la, b_dev, lb, Cs, lc)
+    fwd.launch()
+    torch.cuda.synchronize()
+    Gs = [2 * c for c in Cs]
+    GA = [torch.zeros_like(a) for a in a_dev]
+    GB = [torch.zeros_like(b) for b in b_dev]
+    vja = t1m_amd.Batch(Gs, lc, b_dev, lb, GA, la, conj_b=True)
+    vjb = t1m_amd.Batch(a_dev, la, Gs, lc, GB, lb, conj_a=True)
+    vja.launch()
+    vjb.launch()
+    torch.cuda.synchronize()
+    for bt in (fwd, vja, vjb):
+        bt.close()
+    for i, (A, B) in enumerate(zip(As, Bs)):
+        A2, B2 = A.astype(np.complex128), B.astype(np.complex128)
+        C = oracle_einsum(A2, la, B2, lb, np.zeros(Cs[i].shape, np.complex128, order="F"), lc)
+        ga_want = oracle_einsum(2 * C, lc, B2, lb, np.zeros_like(A2, order="F"), la, conj_b=True)
+        gb_want = oracle_einsum(A2, la, 2 * C, lc, np.zeros_like(B2, order="F"), lb, conj_a=True)
+        assert rel_err(to_host(GA[i]), ga_want) < 1e-5, i
+        assert rel_err(to_host(GB[i]), gb_want) < 1e-5, i
+
+        def loss(Ax, Bx):
+            return float(np.sum(np.abs(np.einsum(f"{la},{lb}->{lc}", Ax, Bx)) ** 2))
+
+        h = 1e-6
+        ga, gb = to_host(GA[i]), to_host(GB[i])
+        for _ in range(2):
+            ia = tuple(int(rng.integers(0, s)) for s in A2.shape)
+            ib = tuple(int(rng.integers(0, s)) for s in B2.shape)
+            for unit, part in ((1.0, "re"), (1j, "im")):
+                Ap, Am = A2.copy(), A2.copy()
+                Ap[ia] += unit * h
+                Am[ia] -= unit * h
+                fd = (loss(Ap, B2) - loss(Am, B2)) / (2 * h)
+                an = ga[ia].real if part == "re" else ga[ia].imag
+                assert abs(fd - an) <= 1e-4 * max(1.0, abs(fd)), (i, ia, part, fd, an)
+                Bp, Bm = B2.copy(), B2.copy()
+                Bp[ib] += unit * h
+                Bm[ib] -= unit * h
+                fd = (loss(A2, Bp) - loss(A2, Bm)) / (2 * h)
+                an = gb[ib].real if part == "re" else gb[ib].imag
+                assert abs(fd - an) <= 1e-4 * max(1.0, abs(fd)), (i, ib, part, fd, an)
+
+
 def test_batch_matches_single_calls():
     rng = np.random.default_rng(15)
     la, lb, lc = "asb", "btc", "astc"

@@ -151,6 +151,46 @@ void suite(size_t max_d) {
   gemm_like<T, 4, 4, 4>("abcd", "cdef", "abef", 2, 2, 2, max_d);
 }
 
+// contract_vjp (this header's addition): grad_a += grad_c . conj(B) and
+// grad_b += conj(A) . grad_c for 'asb,btc->astc', against the naive loop.
+template <typename T>
+void vjp_case(double tol) {
+  std::mt19937_64 g(77);
+  const std::array<size_t, 3> da{5, 4, 6}, db{6, 4, 3};
+  const std::array<size_t, 4> dc{5, 4, 4, 3};
+  std::vector<T> A(120), B(72), G(240), GA(120), GB(72);
+  for (auto& x : A) x = rnd<T>(g);
+  for (auto& x : B) x = rnd<T>(g);
+  for (auto& x : G) x = rnd<T>(g);
+  for (auto& x : GA) x = rnd<T>(g);
+  for (auto& x : GB) x = rnd<T>(g);
+  std::vector<T> cA(A.size()), cB(B.size());
+  for (size_t i = 0; i < A.size(); ++i) cA[i] = std::conj(A[i]);
+  for (size_t i = 0; i < B.size(); ++i) cB[i] = std::conj(B[i]);
+  std::vector<std::complex<double>> RA(GA.begin(), GA.end()), RB(GB.begin(), GB.end());
+  naive(G, {dc.begin(), dc.end()}, "astc", cB, {db.begin(), db.end()}, "btc", RA,
+        {da.begin(), da.end()}, "asb");
+  naive(cA, {da.begin(), da.end()}, "asb", G, {dc.begin(), dc.end()}, "astc", RB,
+        {db.begin(), db.end()}, "btc");
+  t1m::tensor<T, 3> a(da, A.data()), b(db, B.data()), ga(da, GA.data()), gb(db, GB.data());
+  t1m::tensor<T, 4> gc(dc, G.data());
+  t1m::contract_vjp(a, "asb", b, "btc", gc, "astc", &ga, &gb);
+  auto rel = [](const std::vector<T>& x, const std::vector<std::complex<double>>& r) {
+    double num = 0, den = 0;
+    for (size_t i = 0; i < x.size(); ++i) {
+      num += std::norm(std::complex<double>(x[i]) - r[i]);
+      den += std::norm(r[i]);
+    }
+    return std::sqrt(num / den);
+  };
+  const double ea = rel(GA, RA), eb = rel(GB, RB);
+  const bool ok = ea <= tol && eb <= tol;
+  g_fail += ok ? 0 : 1;
+  std::printf("%-4s contract_vjp asb,btc->astc %s rel_err=%.3e,%.3e tol=%.1e\n",
+              ok ? "ok" : "FAIL", sizeof(typename T::value_type) == 4 ? "c64" : "c128", ea, eb,
+              tol);
+}
+
 }  // namespace
 
 int main() {
@@ -165,6 +205,8 @@ int main() {
                              {5, 6, 4, 3}, {3, 7, 6, 2}, {5, 4, 2, 7}, 1e-5);
   check_case<double, 5, 2, 5>("efbad,cf->abcde f64", "efbad", "cf", "abcde",
                               {4, 3, 5, 2, 3}, {6, 3}, {2, 5, 6, 3, 4}, 1e-12);
+  vjp_case<std::complex<float>>(1e-5);
+  vjp_case<std::complex<double>>(1e-12);
   suite<float>(12);
   suite<double>(12);
   suite<std::complex<float>>(12);

@@ -988,3 +988,66 @@ def test_thin_m_runs_transposed(dtype):
     with forced(None):
         assert t1m_amd.plan_kernel(mk("ea", ext), "ea", mk("ebcd", ext), "ebcd",
                                    mk("abcd", ext), "abcd") == thin
+
+
+def test_concurrent_calls_from_threads():
+    """Reentrancy: four host threads, each on a stream of its own, issue
+    contractions of shared and distinct shapes (new plans, cached plans,
+    the autotuner's first call) while a fifth thread keeps clearing the plan
+    cache; every result matches a single-threaded call."""
+    import threading
+    import time
+    rng = np.random.default_rng(41)
+    problems = []
+    for la, lb, lc, ext in [("abcd", "dbef", "acef", dict(a=7, b=5, c=9, d=6, e=8, f=4)),
+                            ("asb", "btc", "astc", dict(a=33, s=4, b=21, t=4, c=17)),
+                            ("ab", "bc", "ac", dict(a=130, b=70, c=150)),
+                            ("efbad", "cf", "abcde", dict(a=6, b=5, c=11, d=4, e=7, f=9))]:
+        for dtype in (np.complex64, np.complex128):
+            A = rand(rng, [ext[x] for x in la], dtype)
+            B = rand(rng, [ext[x] for x in lb], dtype)
+            C0 = rand(rng, [ext[x] for x in lc], dtype)
+            problems.append((la, lb, lc, to_dev(A), to_dev(B), C0))
+    want = []
+    for la, lb, lc, a, b, C0 in problems:
+        c = to_dev(C0)
+        t1m_amd.contract(a, la, b, lb, c, lc)
+        want.append(to_host(c))
+    torch.cuda.synchronize()
+    t1m_amd.plan_cache_clear()
+    errors = []
+    stop = threading.Event()
+
+    def worker(w):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for it in range(24):
+                    i = (3 * w + it) % len(problems)
+                    la, lb, lc, a, b, C0 = problems[i]
+                    c = to_dev(C0)
+                    t1m_amd.contract(a, la, b, lb, c, lc, autotune=(it % 5 == 0))
+                    err = rel_err(to_host(c) - C0, want[i] - C0)
+                    tol = TOL["fp32"] if c.dtype == torch.complex64 else TOL["fp64"]
+                    if not err < tol:
+                        errors.append((w, it, i, err))
+        except Exception as e:  # reported below, on the main thread
+            errors.append((w, repr(e)))
+
+    def clearer():
+        while not stop.is_set():
+            t1m_amd.plan_cache_clear()
+            time.sleep(0.002)
+
+    threads = [threading.Thread(target=worker, args=(w,)) for w in range(4)]
+    cl = threading.Thread(target=clearer)
+    cl.start()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=90)
+    stop.set()
+    cl.join(timeout=10)
+    torch.cuda.synchronize()
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors[:5]

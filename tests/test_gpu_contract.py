@@ -1051,3 +1051,37 @@ def test_concurrent_calls_from_threads():
     torch.cuda.synchronize()
     assert not any(t.is_alive() for t in threads)
     assert not errors, errors[:5]
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_contract_captured_in_a_caller_graph(dtype):
+    """A caller captures t1m_contract (cached plan, asynchronous) into its
+    own HIP graph: every replay adds A.B to C again, with eager work and
+    synchronizations between the replays; a K small enough for split K
+    (atomic adds into C) is included."""
+    rng = np.random.default_rng(52)
+    cases = [("abcd", "dbef", "acef", dict(a=12, b=8, c=12, d=8, e=16, f=8)),
+             ("asb", "btc", "astc", dict(a=40, s=4, b=2048, t=4, c=24))]
+    for la, lb, lc, ext in cases:
+        A = rand(rng, [ext[x] for x in la], dtype)
+        B = rand(rng, [ext[x] for x in lb], dtype)
+        C0 = rand(rng, [ext[x] for x in lc], dtype)
+        prod = oracle_einsum(A, la, B, lb, np.zeros_like(C0, order="F"), lc)
+        a, b, c = to_dev(A), to_dev(B), to_dev(C0)
+        t1m_amd.contract(a, la, b, lb, c, lc)          # builds the plan eagerly
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            t1m_amd.contract(a, la, b, lb, c, lc, synchronize=False)
+        tol = TOL["fp32"] if dtype == np.complex64 else TOL["fp64"]
+        base = to_host(c)                               # after the eager call only
+        for r in range(3):
+            g.replay()
+            torch.cuda.synchronize()
+            got = to_host(c)
+            assert rel_err(got - base, prod) < tol, (la, r)
+            base = got
+            scratch = to_dev(C0)                        # eager work between replays
+            t1m_amd.contract(a, la, b, lb, scratch, lc)
+            assert rel_err(to_host(scratch) - C0, prod) < tol, (la, r)
+        del g

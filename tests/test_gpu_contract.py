@@ -1085,3 +1085,28 @@ def test_contract_captured_in_a_caller_graph(dtype):
             t1m_amd.contract(a, la, b, lb, scratch, lc)
             assert rel_err(to_host(scratch) - C0, prod) < tol, (la, r)
         del g
+
+
+def test_offsets_beyond_int32_take_64bit_maps():
+    """A strided view whose element offsets pass 2^31 (a 25 GB buffer): the
+    plan switches to 64-bit maps by itself (the int32 maps would wrap);
+    checked against a complex128 product of a dense copy, both directions
+    of the operand roles."""
+    step = 1 << 26                                   # 47 * 2^26 > 2^31 - 1
+    buf = torch.zeros(47 * step + 64, dtype=torch.complex64, device="cuda")
+    try:
+        big = torch.as_strided(buf, (64, 48), (1, step))
+        big.copy_(torch.randn(64, 48, dtype=torch.complex64, device="cuda"))
+        B = torch.randn(48, 40, dtype=torch.complex64, device="cuda")
+        C = torch.zeros(64, 40, dtype=torch.complex64, device="cuda")
+        t1m_amd.contract(big, "ik", B, "kj", C, "ij")
+        ref = big.contiguous().to(torch.complex128) @ B.to(torch.complex128)
+        assert rel_err(C.cpu().numpy(), ref.cpu().numpy()) < TOL["fp32"]
+        A = torch.randn(40, 64, dtype=torch.complex64, device="cuda")
+        C2 = torch.zeros(40, 48, dtype=torch.complex64, device="cuda")
+        t1m_amd.contract(A, "ik", big, "kj", C2, "ij")
+        ref2 = A.to(torch.complex128) @ big.contiguous().to(torch.complex128)
+        assert rel_err(C2.cpu().numpy(), ref2.cpu().numpy()) < TOL["fp32"]
+    finally:
+        del buf
+        torch.cuda.empty_cache()

@@ -123,3 +123,29 @@ def test_view_and_extent_validation():
     modes = (ctypes.c_int64 * 2)(0, 1)
     assert lib.t1m_index_maps(ctypes.byref(t), modes, 2, 4, one, None, None) == 1
     assert b"negative extent" in lib.t1m_last_error()
+
+
+def test_product_code_never_uses_the_oracle():
+    # The oracle is test infrastructure: the package, its native sources and
+    # the public headers neither import, link nor load it, so no product path
+    # can fall back to the CPU restatement.
+    pattern = re.compile(r"oracle", re.I)
+    roots = [os.path.join(ROOT, "t1m_amd"), os.path.join(ROOT, "include")]
+    hits = []
+    for root in roots:
+        for dirpath, _, files in os.walk(root):
+            if "_build" in dirpath or "__pycache__" in dirpath:
+                continue
+            for f in files:
+                if f.endswith((".py", ".hip", ".h", ".cpp", ".hpp")) or f == "Makefile":
+                    with open(os.path.join(dirpath, f), errors="replace") as fh:
+                        if pattern.search(fh.read()):
+                            hits.append(os.path.join(dirpath, f))
+    assert not hits, hits
+    # bench.py touches it only inside its CPU-baseline functions
+    with open(os.path.join(ROOT, "bench.py")) as fh:
+        src = fh.read()
+    for m in re.finditer(r"from oracle import", src):
+        head = src[:m.start()]
+        fn = head[head.rfind("\ndef "):].split("(")[0].strip()
+        assert fn in ("def cpu_baseline", "def c5_cpu_baseline"), fn

@@ -149,3 +149,18 @@ def test_product_code_never_uses_the_oracle():
         head = src[:m.start()]
         fn = head[head.rfind("\ndef "):].split("(")[0].strip()
         assert fn in ("def cpu_baseline", "def c5_cpu_baseline"), fn
+
+
+def test_missing_library_fails_loudly():
+    # No fallback: without the built HIP library the package refuses to run.
+    code = ("import t1m_amd, sys\n"
+            "try:\n"
+            "    t1m_amd.load('/nonexistent/libt1m_hip.so')\n"
+            "except OSError as e:\n"
+            "    print('refused:', e); sys.exit(0)\n"
+            "sys.exit(1)\n")
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0 and "is missing" in r.stdout, (r.stdout, r.stderr)

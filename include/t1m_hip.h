@@ -103,11 +103,15 @@ int t1m_contract(const t1m_tensor* a, const char* la, const t1m_tensor* b,
                  const t1m_options* opt);
 
 /*
- * Vector-Jacobian product of C = A . B (no counterpart in the reference):
- *   grad_a += grad_c . conj(B)   (contracted over B's free modes)
- *   grad_b += conj(A) . grad_c   (contracted over A's free modes)
- * grad_a / grad_b may be NULL to skip one.  For real dtypes conj is identity.
- * With torch's convention, grad_c of a real loss is dL/dRe + i dL/dIm.
+ * Vector-Jacobian product of C = alpha * A' . B' (no counterpart in the
+ * reference), where A' = conj(A) if opt->conj_a is set, else A (B' likewise),
+ * exactly the operands t1m_contract uses with the same options:
+ *   grad_a += conj(alpha) * grad_c . conj(B')   (over B's free modes)
+ *   grad_b += conj(alpha) * conj(A') . grad_c   (over A's free modes)
+ * The gradients are with respect to A' and B' (what a caller holding a lazy
+ * conjugate view differentiates).  grad_a / grad_b may be NULL to skip one.
+ * For real dtypes conj is identity.  With torch's convention, grad_c of a
+ * real loss is dL/dRe + i dL/dIm.
  */
 int t1m_contract_vjp(const t1m_tensor* a, const char* la, const t1m_tensor* b,
                      const char* lb, const t1m_tensor* grad_c, const char* lc,

@@ -189,6 +189,30 @@ def _stream_handle(stream) -> Optional[int]:
     return stream.cuda_stream or None
 
 
+def _operand(t):
+    """An input tensor as (tensor over plain memory, conjugated?).
+
+    torch marks some views lazily: a conj view (``x.conj()``, and the
+    grad_output autograd hands to a backward through ``C.conj()``) shares x's
+    memory with a conjugation bit, and a neg view holds the negated value.
+    The library reads raw memory, so a conj bit becomes the call's conj flag
+    (its memory is passed as is) and a neg bit is materialised."""
+    if t.is_neg():
+        t = t.resolve_neg()
+    if t.is_conj():
+        return t.conj(), True     # clears the bit: same memory, unconjugated
+    return t, False
+
+
+def _output(t, what: str):
+    """An output tensor (accumulated in place) must hold its plain value."""
+    if t.is_conj() or t.is_neg():
+        raise ValueError(f"{what} is a lazy conj/neg view of another tensor; the "
+                         "library would add into the unconjugated memory "
+                         "(pass a resolved tensor)")
+    return t
+
+
 def contract(a, la: str, b, lb: str, c, lc: str, alpha=1.0, precision="native",
              conj_a: bool = False, conj_b: bool = False, stream=None,
              synchronize: bool = True, autotune: bool = False,
@@ -196,12 +220,17 @@ def contract(a, la: str, b, lb: str, c, lc: str, alpha=1.0, precision="native",
     """C += alpha * A . B over labels (t1m::contract, t1m.h:560-577).
 
     a, b, c are torch tensors of one dtype (float32/64, complex64/128) with any
-    strides; device tensors run without copies.  autotune=True times the
-    candidate tile shapes the first time a shape is seen; deterministic=True
-    forbids splitting K (bitwise-reproducible results).  Returns c."""
+    strides; device tensors run without copies.  Lazy conj views of a or b are
+    read through the conj flags (no copy); c must not be a conj or neg view.
+    autotune=True times the candidate tile shapes the first time a shape is
+    seen; deterministic=True forbids splitting K (bitwise-reproducible
+    results).  Returns c."""
+    a, ca = _operand(a)
+    b, cb = _operand(b)
+    _output(c, "c")
     da, db, dc = _Desc(a), _Desc(b), _Desc(c)
-    o = _options(alpha, precision, conj_a, conj_b, stream, synchronize, autotune,
-                 deterministic)
+    o = _options(alpha, precision, bool(conj_a) ^ ca, bool(conj_b) ^ cb, stream,
+                 synchronize, autotune, deterministic)
     _check(load().t1m_contract(ctypes.byref(da.desc), la.encode(), ctypes.byref(db.desc),
                                lb.encode(), ctypes.byref(dc.desc), lc.encode(),
                                ctypes.byref(o)))
@@ -210,11 +239,24 @@ def contract(a, la: str, b, lb: str, c, lc: str, alpha=1.0, precision="native",
 
 def contract_vjp(a, la: str, b, lb: str, grad_c, lc: str, grad_a=None, grad_b=None,
                  alpha=1.0, precision="native", stream=None, synchronize: bool = True):
-    """grad_a += conj(alpha) grad_c . conj(B);  grad_b += conj(alpha) conj(A) . grad_c."""
+    """grad_a += conj(alpha) grad_c . conj(B);  grad_b += conj(alpha) conj(A) . grad_c.
+
+    Gradients are with respect to the values of a and b as torch sees them:
+    a lazy conj view of a or b is passed as its memory with a conj flag, a
+    lazy conj view of grad_c is materialised, and grad_a / grad_b must not be
+    conj or neg views."""
+    a, ca = _operand(a)
+    b, cb = _operand(b)
+    if grad_c.is_conj() or grad_c.is_neg():
+        grad_c = grad_c.resolve_conj().resolve_neg()
+    if grad_a is not None:
+        _output(grad_a, "grad_a")
+    if grad_b is not None:
+        _output(grad_b, "grad_b")
     da, db, dg = _Desc(a), _Desc(b), _Desc(grad_c)
     ga = _Desc(grad_a) if grad_a is not None else None
     gb = _Desc(grad_b) if grad_b is not None else None
-    o = _options(alpha, precision, False, False, stream, synchronize)
+    o = _options(alpha, precision, ca, cb, stream, synchronize)
     _check(load().t1m_contract_vjp(
         ctypes.byref(da.desc), la.encode(), ctypes.byref(db.desc), lb.encode(),
         ctypes.byref(dg.desc), lc.encode(),
@@ -232,23 +274,44 @@ class Batch:
         n = len(a)
         if not (len(b) == n and len(c) == n):
             raise ValueError("a, b and c must have the same length")
+        a, ca = self._operands(a)
+        b, cb = self._operands(b)
+        c = [_output(z, "c") for z in c]
+        # The captured graph holds these tensors' device addresses: keep the
+        # tensors alive (and their memory out of torch's allocator) until
+        # close().  C-ABI callers of t1m_batch_create keep the same rule.
+        self._tensors = (a, b, c)
         self._keep = [(_Desc(x), _Desc(y), _Desc(z)) for x, y, z in zip(a, b, c)]
         arr = lambda k: (TensorDesc * max(n, 1))(*[d[k].desc for d in self._keep])
         self._a, self._b, self._c = arr(0), arr(1), arr(2)
-        o = _options(alpha, precision, conj_a, conj_b, stream, synchronize)
+        o = _options(alpha, precision, bool(conj_a) ^ ca, bool(conj_b) ^ cb, stream,
+                     synchronize)
         h = ctypes.c_void_p()
         _check(load().t1m_batch_create(n, self._a, la.encode(), self._b, lb.encode(),
                                        self._c, lc.encode(), ctypes.byref(o),
                                        ctypes.byref(h)))
         self._h = h
 
+    @staticmethod
+    def _operands(ts):
+        """Inputs of every instance, and one conj flag for the whole batch:
+        when all instances are lazy conj views their memory is read with the
+        flag set; when only some are, those are materialised."""
+        ops = [_operand(t) for t in ts]
+        if ops and all(cj for _, cj in ops):
+            return [t for t, _ in ops], True
+        return [t.conj().resolve_conj() if cj else t for t, cj in ops], False
+
     def launch(self, stream=None) -> None:
+        if not getattr(self, "_h", None):
+            raise RuntimeError("launch() after close()")
         _check(load().t1m_batch_launch(self._h, _stream_handle(stream)))
 
     def close(self) -> None:
         if getattr(self, "_h", None):
             load().t1m_batch_destroy(self._h)
             self._h = None
+        self._tensors = None
 
     def __del__(self):
         try:

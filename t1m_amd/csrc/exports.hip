@@ -19,11 +19,17 @@ using namespace t1m_hip;
 namespace {
 
 // A view's counts and arrays: non-negative counts, and every array the
-// kernels index present when the view is not empty.
-bool view_ok(const t1m_view& v) {
+// kernels index present when the view is not empty.  pack_block reads the
+// block strides of every block its rows / columns start (rbs[r0 / br] for
+// r0 < nrs), so with `whole_blocks` the block-scatter vectors must cover
+// ceil(nrs / br) and ceil(ncs / bc) entries; unpack reads entry 0 only.
+bool view_ok(const t1m_view& v, bool whole_blocks) {
   if (v.nrs < 0 || v.ncs < 0 || v.nrbs < 0 || v.ncbs < 0) return false;
   if (v.nrs == 0 || v.ncs == 0) return true;
-  return v.rs && v.cs && v.rbs && v.cbs && v.nrbs > 0 && v.ncbs > 0;
+  if (!(v.rs && v.cs && v.rbs && v.cbs && v.nrbs > 0 && v.ncbs > 0)) return false;
+  if (!whole_blocks) return true;
+  if (v.br <= 0 || v.bc <= 0) return false;
+  return v.nrbs >= (v.nrs + v.br - 1) / v.br && v.ncbs >= (v.ncs + v.bc - 1) / v.bc;
 }
 
 }  // namespace
@@ -129,7 +135,7 @@ int t1m_pack_block(int32_t dtype, int32_t label, const t1m_view* view,
   const bool cplx = dtype == T1M_COMPLEX64 || dtype == T1M_COMPLEX128;
   if (view->br <= 0 || view->bc <= 0 || (cplx && (view->br < 2 || (label == 0 && view->bc < 2))))
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "block sizes too small for the panel format");
-  if (!view_ok(*view) || length < 0)
+  if (!view_ok(*view, true) || length < 0)
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "bad view counts or arrays");
   HIP_TRY(launch_pack_block(dtype, label, *view, length, src, dest,
                             static_cast<hipStream_t>(stream)));
@@ -143,7 +149,7 @@ int t1m_unpack(int32_t dtype, const t1m_view* view, const void* src,
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "null argument");
   if (dtype_size(dtype) == 0)
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
-  if (!view_ok(*view) || view->br <= 0)
+  if (!view_ok(*view, false) || view->br <= 0)
     return set_error(T1M_ERROR_INVALID_ARGUMENT, "bad view counts, arrays or block size");
   HIP_TRY(launch_unpack(dtype, *view, src, dest, static_cast<hipStream_t>(stream)));
   return T1M_SUCCESS;

@@ -95,6 +95,8 @@ int get_plan(const Problem& P, int precision, hipStream_t stream,
              std::shared_ptr<Plan>& out);
 int64_t plan_cache_size();
 void plan_cache_clear();
+// While `stream` is capturing, ties the plan's lifetime to the graph.
+int retain_plan_for_capture(const std::shared_ptr<Plan>& plan, hipStream_t stream);
 bool is_device_pointer(const void* ptr);
 // Kernel ids worth timing for a plan's dtype and precision.
 std::vector<int> kernel_candidates(int dtype, int precision);

@@ -322,6 +322,16 @@ void append_tune_cache(TuneCache& T, const std::string& key, int kernel) {
   fclose(f);
 }
 
+// A kernel id read from the tuning file is used only if it computes the
+// plan's dtype at its precision and the plan's K fits the MFMA kernels' int
+// k (a stale or edited file must not bind, say, an FP64 kernel to complex
+// float data).
+bool tuned_kernel_ok(const Plan& pl, int kernel) {
+  if (kernel < 0 || kernel >= K_COUNT || !kernel_fits(kernel, pl.dtype, pl.precision))
+    return false;
+  return kernel == K_SIMPLE || pl.K < (int64_t(1) << 31);
+}
+
 std::string tune_key(const Plan& pl) {
   int dev = 0;
   (void)hipGetDevice(&dev);
@@ -360,7 +370,7 @@ int autotune_plan(Plan& pl, const GemmProblem& g0, hipStream_t stream) {
     std::lock_guard<std::mutex> lock(T.mu);
     load_tune_cache(T);
     auto it = T.best.find(key);
-    if (it != T.best.end()) {
+    if (it != T.best.end() && tuned_kernel_ok(pl, it->second)) {
       pl.kernel = it->second;
       pl.tuned = true;
       return T1M_SUCCESS;
@@ -437,6 +447,31 @@ PlanCache& cache() {
   static PlanCache* c = new PlanCache();   // never destroyed (no HIP at exit)
   return *c;
 }
+
+// Plans that a captured graph held until it was destroyed.  The graph's
+// user-object destructor may run on a runtime thread where HIP calls are not
+// allowed, so it only parks its reference here; the next plan creation or
+// cache clear drops the parked references (freeing the plans' device memory
+// when nothing else holds them).
+std::mutex g_parked_mu;
+std::vector<std::shared_ptr<Plan>>* g_parked = new std::vector<std::shared_ptr<Plan>>();
+
+void park_plan_ref(void* p) {
+  auto* ref = static_cast<std::shared_ptr<Plan>*>(p);
+  {
+    std::lock_guard<std::mutex> lock(g_parked_mu);
+    g_parked->push_back(std::move(*ref));
+  }
+  delete ref;
+}
+
+void drop_parked_plans() {
+  std::vector<std::shared_ptr<Plan>> dead;
+  {
+    std::lock_guard<std::mutex> lock(g_parked_mu);
+    dead.swap(*g_parked);
+  }
+}   // `dead` releases its plans here, outside the lock
 
 std::string plan_key(const Problem& P, int precision, int device) {
   std::string k;
@@ -603,6 +638,7 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
       return T1M_SUCCESS;
     }
   }
+  drop_parked_plans();
   auto plan = std::make_shared<Plan>();
   Problem P = P0;
   Plan& pl = *plan;
@@ -720,7 +756,7 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
     std::lock_guard<std::mutex> lock(T.mu);
     load_tune_cache(T);
     auto it = T.best.find(tune_key(pl));
-    if (it != T.best.end()) {
+    if (it != T.best.end() && tuned_kernel_ok(pl, it->second)) {
       pl.kernel = it->second;
       pl.tuned = true;
     }
@@ -758,11 +794,52 @@ std::atomic<uint64_t> g_plan_generation{0};
 }  // namespace
 
 void plan_cache_clear() {
-  PlanCache& C = cache();
-  std::lock_guard<std::mutex> lock(C.mu);
-  C.map.clear();
-  C.lru.clear();
-  g_plan_generation.fetch_add(1);
+  {
+    PlanCache& C = cache();
+    std::lock_guard<std::mutex> lock(C.mu);
+    C.map.clear();
+    C.lru.clear();
+    g_plan_generation.fetch_add(1);
+  }
+  {
+    TuneCache& T = tune_cache();   // re-read the tuning file on next use
+    std::lock_guard<std::mutex> lock(T.mu);
+    T.loaded = false;
+    T.best.clear();
+  }
+  drop_parked_plans();
+}
+
+// A call captured into a graph bakes the plan's map pointers (and batch
+// offsets) into the graph's kernel nodes.  The graph then holds a reference
+// to the plan (a user object released when the graph and its executable
+// graphs are destroyed), so evicting or clearing the plan cache cannot free
+// memory a later replay reads.
+int retain_plan_for_capture(const std::shared_ptr<Plan>& plan, hipStream_t stream) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &st) != hipSuccess) {
+    (void)hipGetLastError();   // e.g. the null stream beside a global capture:
+    return T1M_SUCCESS;        // the launch itself reports that case
+  }
+  if (st != hipStreamCaptureStatusActive) return T1M_SUCCESS;
+  hipGraph_t graph = nullptr;
+  unsigned long long id = 0;
+  HIP_TRY(hipStreamGetCaptureInfo_v2(stream, &st, &id, &graph, nullptr, nullptr));
+  if (!graph) return set_error(T1M_ERROR_DEVICE, "capturing stream without a graph");
+  auto* ref = new std::shared_ptr<Plan>(plan);
+  hipUserObject_t obj = nullptr;
+  hipError_t e = hipUserObjectCreate(&obj, ref, park_plan_ref, 1,
+                                     hipUserObjectNoDestructorSync);
+  if (e != hipSuccess) {
+    delete ref;
+    return set_error(T1M_ERROR_DEVICE, "hipUserObjectCreate: %s", hipGetErrorString(e));
+  }
+  e = hipGraphRetainUserObject(graph, obj, 1, hipGraphUserObjectMove);
+  if (e != hipSuccess) {
+    (void)hipUserObjectRelease(obj, 1);   // runs park_plan_ref
+    return set_error(T1M_ERROR_DEVICE, "hipGraphRetainUserObject: %s", hipGetErrorString(e));
+  }
+  return T1M_SUCCESS;
 }
 
 // ---------------------------------------------------------------------------
@@ -945,6 +1022,7 @@ int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
   }
   if (plan->swap_ab) transpose_problem(g);
   if (opt.autotune && !plan->tuned.load() && (rc = autotune_plan(*plan, g, stream))) return rc;
+  if ((rc = retain_plan_for_capture(plan, stream))) return rc;
   hipError_t e = launch_contract(g, plan->kernel, stream);
   if (e != hipSuccess)
     return set_error(T1M_ERROR_DEVICE, "kernel %s launch failed: %s",
@@ -1025,17 +1103,21 @@ int t1m_contract_vjp(const t1m_tensor* a, const char* la, const t1m_tensor* b,
   Problem P;
   int rc = parse_problem(a, la, b, lb, grad_c, lc, P);
   if (rc) return rc;
-  // d/dA of alpha*A.B paired with grad_c: conj(alpha) * grad_c . conj(B).
+  // The forward is C += alpha * A' . B' with A' = conj(A) when conj_a is set
+  // (B' likewise); gradients are taken with respect to A' and B':
+  //   grad_a += conj(alpha) * grad_c . conj(B'),  conj(B') = B if conj_b;
+  //   grad_b += conj(alpha) * conj(A') . grad_c.
+  const bool cplx = P.dtype == T1M_COMPLEX64 || P.dtype == T1M_COMPLEX128;
   t1m_options g = o;
   g.alpha_im = -o.alpha_im;
-  g.conj_a = 0;
   if (grad_a) {
-    g.conj_b = 1;
+    g.conj_a = 0;
+    g.conj_b = cplx && !o.conj_b;
     if ((rc = contract_impl(grad_c, lc, b, lb, grad_a, la, g))) return rc;
   }
   if (grad_b) {
+    g.conj_a = cplx && !o.conj_a;
     g.conj_b = 0;
-    g.conj_a = 1;
     if ((rc = contract_impl(a, la, grad_c, lc, grad_b, lb, g))) return rc;
   }
   return T1M_SUCCESS;

@@ -118,6 +118,15 @@ def test_view_and_extent_validation():
     assert lib.t1m_pack_block(t1m_amd.FLOAT32, 1, ctypes.byref(no_blocks), 8, one, one, None) == 1
     bad_ld = t1m_amd.View(0, 4, one, 3, one, 2, one, 1, one, 1)
     assert lib.t1m_unpack(t1m_amd.FLOAT32, ctypes.byref(bad_ld), one, one, None) == 1
+    # pack_block reads the block stride of every block: 9 rows at br = 4 need
+    # ceil(9/4) = 3 row-block entries, 5 columns at bc = 2 need 3 column ones.
+    for dt in (t1m_amd.FLOAT32, t1m_amd.COMPLEX64):
+        for lab in (0, 1):
+            short_rbs = t1m_amd.View(4, 2, one, 9, one, 5, one, 2, one, 3)
+            assert lib.t1m_pack_block(dt, lab, ctypes.byref(short_rbs), 8, one, one, None) == 1
+            assert b"view" in lib.t1m_last_error()
+            short_cbs = t1m_amd.View(4, 2, one, 9, one, 5, one, 3, one, 2)
+            assert lib.t1m_pack_block(dt, lab, ctypes.byref(short_cbs), 8, one, one, None) == 1
     dims = (ctypes.c_int64 * 2)(3, -2)
     t = t1m_amd.TensorDesc(t1m_amd.FLOAT32, 2, dims, None, None)
     modes = (ctypes.c_int64 * 2)(0, 1)

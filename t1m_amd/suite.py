@@ -6,7 +6,9 @@ Runs the 24 contractions of the reference's benchmark (benchmark/benchmark.cpp:
 the whole contract call timed, minimum in microseconds (benchmark.cpp:7,
 64-79).  Here the tensors are device-resident and the call returns after the
 result is visible.  Prints the reference's CSV (`labels,min`) with extra
-columns: TFLOP/s and the kernel used.
+columns: TFLOP/s, the kernel used, and the first (cold) call, which also
+builds the plan (the reference builds bundles, scatter vectors and its
+workspace inside every call, benchmark.cpp:68-75).
 
     python -m t1m_amd.suite [--dtypes f32,f64,c64,c128] [--repeats 10] [--autotune]
 """
@@ -74,23 +76,27 @@ def main(argv=None) -> int:
     for name in args.dtypes.split(","):
         dt = dts[name]
         print(f"# {name}")
-        print("contraction,min,tflops,kernel", flush=True)
+        print("contraction,min,tflops,kernel,first_call", flush=True)
         for dims_a, la, dims_b, lb, dims_c, lc in SUITE:
             def colmajor(shape):
                 t = torch.rand(list(reversed(shape)), dtype=dt, device="cuda")
                 return t.permute(*reversed(range(len(shape))))
             a, b, c = colmajor(dims_a), colmajor(dims_b), colmajor(dims_c)
-            best = float("inf")
-            for _ in range(args.repeats):
+            best = first = float("inf")
+            for r in range(args.repeats):
                 c.zero_()
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 t1m.contract(a, la, b, lb, c, lc, autotune=args.autotune)
-                best = min(best, time.perf_counter() - t0)
+                dt_s = time.perf_counter() - t0
+                if r == 0:
+                    first = dt_s
+                best = min(best, dt_s)
             us = best * 1e6
             fl = flops(dims_a, la, lb, dims_c, lc, dt.is_complex)
             kern = t1m.plan_kernel(a, la, b, lb, c, lc)
-            print(f"{la}-{lb}-{lc},{us:.0f},{fl / best / 1e12:.2f},{kern}", flush=True)
+            print(f"{la}-{lb}-{lc},{us:.0f},{fl / best / 1e12:.2f},{kern},{first * 1e6:.0f}",
+                  flush=True)
     return 0
 
 

@@ -56,7 +56,7 @@ struct Plan {
   bool swap_ab = false;                          // run as C^T = B^T A^T (thin m, see get_plan)
   int64_t M = 0, N = 0, K = 0, L = 1;
   int a_mfast = 1, b_kfast = 1, idx32 = 1;
-  void* maps = nullptr;
+  void* maps = nullptr;                          // owned device memory (null: borrowed)
   // Batch problems (L > 1): byte offsets into A, B, C, device [3][L] and host.
   int64_t* boff = nullptr;
   std::vector<int64_t> hboff;
@@ -93,6 +93,18 @@ const char* env_override(const char* name);
 void transpose_problem(GemmProblem& g);
 int get_plan(const Problem& P, int precision, hipStream_t stream,
              std::shared_ptr<Plan>& out);
+// The parts of get_plan: host-side layout (P receives the bundles in
+// traversal order), the device bytes the maps need, and their construction
+// into caller-provided device memory (queued on `stream`, not waited for).
+void plan_layout(const Problem& P0, int precision, Plan& pl, Problem& P);
+size_t plan_device_bytes(const Plan& pl);
+hipError_t plan_build_maps(Plan& pl, const Problem& P, char* base, hipStream_t stream);
+// A tuning-file entry for the plan's key, if it fits the plan (host only).
+void apply_tuning_file(Plan& pl);
+// The kernel arguments of a call of plan `pl` on device data A, B, C.
+GemmProblem gemm_for(const Plan& pl, const void* A, const void* B, void* C,
+                     const t1m_options& opt);
+int cu_count();
 int64_t plan_cache_size();
 void plan_cache_clear();
 // While `stream` is capturing, ties the plan's lifetime to the graph.

@@ -177,8 +177,6 @@ int parse_problem(const t1m_tensor* a, const char* la, const t1m_tensor* b,
 // Kernel choice
 // ---------------------------------------------------------------------------
 
-namespace {
-
 int cu_count() {
   static int n = [] {
     int dev = 0, cus = 0;
@@ -189,8 +187,6 @@ int cu_count() {
   }();
   return n;
 }
-
-}  // namespace
 
 std::vector<int> kernel_candidates(int dtype, int precision) {
   if (precision == T1M_PRECISION_SIMPLE) return {K_SIMPLE};
@@ -623,26 +619,12 @@ bool may_alias(const std::vector<int64_t>& dims, const std::vector<int64_t>& str
   return false;
 }
 
-int get_plan(const Problem& P0, int precision, hipStream_t stream,
-             std::shared_ptr<Plan>& out) {
-  int device = 0;
-  HIP_TRY(hipGetDevice(&device));
-  const std::string key = plan_key(P0, precision, device);
-  PlanCache& C = cache();
-  {
-    std::lock_guard<std::mutex> lock(C.mu);
-    auto it = C.map.find(key);
-    if (it != C.map.end()) {
-      C.lru.splice(C.lru.begin(), C.lru, it->second.second);
-      out = it->second.first;
-      return T1M_SUCCESS;
-    }
-  }
-  drop_parked_plans();
-  auto plan = std::make_shared<Plan>();
-  Problem P = P0;
-  Plan& pl = *plan;
-  pl.M = P.M; pl.N = P.N; pl.K = P.K;
+// Host half of a plan: orientation, kernel choice, gather directions, the
+// traversal order of each bundle and the offset width.  P receives the
+// problem with its bundles in traversal order.
+void plan_layout(const Problem& P0, int precision, Plan& pl, Problem& P) {
+  P = P0;
+  pl.M = P.M; pl.N = P.N; pl.K = P.K; pl.L = P.L;
   pl.dtype = P.dtype;
   pl.precision = precision;
   // Transposed plans, C^T = B^T . A^T (products commute): every kernel sees
@@ -662,6 +644,7 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
     if (pl.swap_ab) std::swap(Q.M, Q.N);
     pl.kernel = choose_kernel(Q, precision);
   }
+  pl.c_lo = pl.c_hi = 0;
   for (size_t i = 0; i < P.dc.size(); ++i) {
     const int64_t span = (P.dc[i] > 0 ? P.dc[i] - 1 : 0) * P.sc[i];
     if (span < 0) pl.c_lo += span; else pl.c_hi += span;
@@ -688,26 +671,36 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
   pl.idx32 = (offA <= lim && offB <= lim && offC <= lim) ? 1 : 0;
   if (const char* f = env_override("T1M_FORCE_IDX64"))
     if (atoi(f) != 0) pl.idx32 = 0;
-  const size_t isz = pl.idx32 ? 4 : 8;
+  pl.modes_m = P.m; pl.modes_n = P.n; pl.modes_k = P.k;
+}
 
-  // One allocation: rsA, rsC (M), csA, rsB (K), csB, csC (N), each 256-B aligned.
+// Device bytes of a plan's maps: rsA, rsC (M entries), csA, rsB (K), csB,
+// csC (N), then the [3][L] batch byte offsets when L > 1, each of the seven
+// parts 256-B aligned.
+size_t plan_device_bytes(const Plan& pl) {
   auto round = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t isz = pl.idx32 ? 4 : 8;
+  size_t total = 2 * (round(size_t(pl.M) * isz) + round(size_t(pl.K) * isz) +
+                      round(size_t(pl.N) * isz));
+  if (pl.L > 1) total += round(3 * size_t(pl.L) * 8);
+  return total;
+}
+
+// Points the plan's maps into `base` (plan_device_bytes bytes of device
+// memory) and queues their construction on `stream`: one launch for the six
+// scatter vectors, one for the batch offsets (scatter vectors of the batch
+// modes with byte strides).  Nothing waits.
+hipError_t plan_build_maps(Plan& pl, const Problem& P, char* base, hipStream_t stream) {
+  auto round = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t isz = pl.idx32 ? 4 : 8;
   const size_t szM = round(size_t(P.M) * isz), szK = round(size_t(P.K) * isz),
                szN = round(size_t(P.N) * isz);
-  const size_t total = 2 * (szM + szK + szN);
-  char* base = nullptr;
-  if (hipMalloc(&base, total) != hipSuccess) {
-    (void)hipGetLastError();
-    return set_error(T1M_ERROR_OUT_OF_MEMORY, "hipMalloc of %zu bytes for index maps failed", total);
-  }
-  pl.maps = base;
   pl.rsA = base;
   pl.rsC = base + szM;
   pl.csA = base + 2 * szM;
   pl.rsB = base + 2 * szM + szK;
   pl.csB = base + 2 * szM + 2 * szK;
   pl.csC = base + 2 * szM + 2 * szK + szN;
-
   MapBatch mb;
   mb.count = 6;
   fill_map(mb.maps[0], P.m, 0, pl.idx32, pl.rsA);
@@ -717,17 +710,57 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
   fill_map(mb.maps[4], P.n, 1, pl.idx32, pl.csB);
   fill_map(mb.maps[5], P.n, 2, pl.idx32, pl.csC);
   hipError_t e = launch_scatter_maps(mb, stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  if (e != hipSuccess) {
-    (void)hipFree(base);
-    pl.maps = nullptr;
-    return set_error(T1M_ERROR_DEVICE, "index map construction failed: %s", hipGetErrorString(e));
+  pl.boff = nullptr;
+  if (e != hipSuccess || P.L <= 1) return e;
+  // Batch problem z reads A, B, C at sum_j digit_j(z) * stride_j * esize
+  // bytes (mixed radix over the batch modes, first mode fastest).
+  char* boff = base + 2 * (szM + szK + szN);
+  pl.boff = reinterpret_cast<int64_t*>(boff);
+  const int64_t es = dtype_size(P.dtype);
+  std::vector<Mode> bytes = P.batch;
+  for (Mode& md : bytes) { md.sa *= es; md.sb *= es; md.sc *= es; }
+  MapBatch ob;
+  ob.count = 3;
+  for (int w = 0; w < 3; ++w) fill_map(ob.maps[w], bytes, w, false, boff + w * size_t(P.L) * 8);
+  return launch_scatter_maps(ob, stream);
+}
+
+int get_plan(const Problem& P0, int precision, hipStream_t stream,
+             std::shared_ptr<Plan>& out) {
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
+  const std::string key = plan_key(P0, precision, device);
+  PlanCache& C = cache();
+  {
+    std::lock_guard<std::mutex> lock(C.mu);
+    auto it = C.map.find(key);
+    if (it != C.map.end()) {
+      C.lru.splice(C.lru.begin(), C.lru, it->second.second);
+      out = it->second.first;
+      return T1M_SUCCESS;
+    }
   }
-  pl.modes_m = P.m; pl.modes_n = P.n; pl.modes_k = P.k;
-  pl.L = P.L;
+  drop_parked_plans();
+  auto plan = std::make_shared<Plan>();
+  Plan& pl = *plan;
+  Problem P;
+  plan_layout(P0, precision, pl, P);
+
+  // One allocation for the maps and the batch offsets.
+  const size_t total = plan_device_bytes(pl);
+  char* base = nullptr;
+  if (hipMalloc(&base, total) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(T1M_ERROR_OUT_OF_MEMORY, "hipMalloc of %zu bytes for index maps failed", total);
+  }
+  pl.maps = base;
+  hipError_t e = plan_build_maps(pl, P, base, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess)
+    return set_error(T1M_ERROR_DEVICE, "index map construction failed: %s", hipGetErrorString(e));
   if (P.L > 1) {
-    // Byte offsets of batch problem z (mixed radix over the batch modes,
-    // first mode fastest).
+    // The same offsets on the host (the persistent batch flattens instances
+    // into one problem per batch index with them).
     const int64_t es = dtype_size(P.dtype);
     pl.hboff.assign(size_t(3 * P.L), 0);
     for (int64_t z = 0; z < P.L; ++z) {
@@ -741,26 +774,8 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
       pl.hboff[size_t(P.L + z)] = ob * es;
       pl.hboff[size_t(2 * P.L + z)] = oc * es;
     }
-    const size_t bytes = pl.hboff.size() * sizeof(int64_t);
-    if (hipMalloc(reinterpret_cast<void**>(&pl.boff), bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      pl.boff = nullptr;
-      return set_error(T1M_ERROR_OUT_OF_MEMORY, "hipMalloc of %zu bytes for batch offsets failed", bytes);
-    }
-    hipError_t eb = hipMemcpy(pl.boff, pl.hboff.data(), bytes, hipMemcpyHostToDevice);
-    if (eb != hipSuccess)
-      return set_error(T1M_ERROR_DEVICE, "batch offsets upload failed: %s", hipGetErrorString(eb));
   }
-  if (!env_override("T1M_FORCE_KERNEL")) {     // a tuning result from an earlier run
-    TuneCache& T = tune_cache();
-    std::lock_guard<std::mutex> lock(T.mu);
-    load_tune_cache(T);
-    auto it = T.best.find(tune_key(pl));
-    if (it != T.best.end() && tuned_kernel_ok(pl, it->second)) {
-      pl.kernel = it->second;
-      pl.tuned = true;
-    }
-  }
+  apply_tuning_file(pl);          // a tuning result from an earlier run
 
   std::lock_guard<std::mutex> lock(C.mu);
   auto it = C.map.find(key);
@@ -779,8 +794,7 @@ int get_plan(const Problem& P0, int precision, hipStream_t stream,
 }
 
 Plan::~Plan() {
-  if (maps) (void)hipFree(maps);
-  if (boff) (void)hipFree(boff);
+  if (maps) (void)hipFree(maps);   // holds the batch offsets too
 }
 
 int64_t plan_cache_size() {
@@ -939,6 +953,45 @@ void call_signature(const t1m_tensor* const* t, const char* const* l, int precis
 
 }  // namespace
 
+GemmProblem gemm_for(const Plan& pl, const void* A, const void* B, void* C,
+                     const t1m_options& opt) {
+  GemmProblem g;
+  g.A = A; g.B = B; g.C = C;
+  g.rsA = pl.rsA; g.csA = pl.csA; g.rsB = pl.rsB; g.csB = pl.csB;
+  g.rsC = pl.rsC; g.csC = pl.csC;
+  g.M = pl.M; g.N = pl.N; g.K = pl.K;
+  g.alpha_re = opt.alpha_re; g.alpha_im = opt.alpha_im;
+  const bool cplx = pl.dtype == T1M_COMPLEX64 || pl.dtype == T1M_COMPLEX128;
+  g.conj_a = cplx && opt.conj_a; g.conj_b = cplx && opt.conj_b;
+  if (!cplx) g.alpha_im = 0.0;
+  g.a_mfast = pl.a_mfast; g.b_kfast = pl.b_kfast;
+  g.idx32 = pl.idx32;
+  g.dtype = pl.dtype;
+  g.split_k = opt.deterministic ? 1 : 0;
+  g.cus = cu_count();
+  g.c_overlap = pl.c_overlap ? 1 : 0;
+  g.L = pl.L;
+  if (pl.L > 1) {
+    g.boffA = pl.boff;
+    g.boffB = pl.boff + pl.L;
+    g.boffC = pl.boff + 2 * pl.L;
+  }
+  if (pl.swap_ab) transpose_problem(g);
+  return g;
+}
+
+void apply_tuning_file(Plan& pl) {
+  if (env_override("T1M_FORCE_KERNEL")) return;
+  TuneCache& T = tune_cache();
+  std::lock_guard<std::mutex> lock(T.mu);
+  load_tune_cache(T);
+  auto it = T.best.find(tune_key(pl));
+  if (it != T.best.end() && tuned_kernel_ok(pl, it->second)) {
+    pl.kernel = it->second;
+    pl.tuned = true;
+  }
+}
+
 int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
                   const char* lb, const t1m_tensor* c, const char* lc,
                   const t1m_options& opt) {
@@ -999,28 +1052,7 @@ int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
   if (hostB) { if ((rc = sb.stage(b->data, P.db, P.sb, es, true, stream))) return rc; dB = sb.device; }
   if (hostC) { if ((rc = sc.stage(c->data, P.dc, P.sc, es, true, stream))) return rc; dC = sc.device; }
 
-  GemmProblem g;
-  g.A = dA; g.B = dB; g.C = dC;
-  g.rsA = plan->rsA; g.csA = plan->csA; g.rsB = plan->rsB; g.csB = plan->csB;
-  g.rsC = plan->rsC; g.csC = plan->csC;
-  g.M = plan->M; g.N = plan->N; g.K = plan->K;
-  g.alpha_re = opt.alpha_re; g.alpha_im = opt.alpha_im;
-  const bool cplx = P.dtype == T1M_COMPLEX64 || P.dtype == T1M_COMPLEX128;
-  g.conj_a = cplx && opt.conj_a; g.conj_b = cplx && opt.conj_b;
-  if (!cplx) g.alpha_im = 0.0;
-  g.a_mfast = plan->a_mfast; g.b_kfast = plan->b_kfast;
-  g.idx32 = plan->idx32;
-  g.dtype = P.dtype;
-  g.split_k = opt.deterministic ? 1 : 0;
-  g.cus = cu_count();
-  g.c_overlap = plan->c_overlap ? 1 : 0;
-  g.L = plan->L;
-  if (plan->L > 1) {
-    g.boffA = plan->boff;
-    g.boffB = plan->boff + plan->L;
-    g.boffC = plan->boff + 2 * plan->L;
-  }
-  if (plan->swap_ab) transpose_problem(g);
+  GemmProblem g = gemm_for(*plan, dA, dB, dC, opt);
   if (opt.autotune && !plan->tuned.load() && (rc = autotune_plan(*plan, g, stream))) return rc;
   if ((rc = retain_plan_for_capture(plan, stream))) return rc;
   hipError_t e = launch_contract(g, plan->kernel, stream);

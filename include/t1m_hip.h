@@ -119,6 +119,25 @@ int t1m_contract_vjp(const t1m_tensor* a, const char* la, const t1m_tensor* b,
                      const t1m_options* opt);
 
 /*
+ * t1m_contract without device allocation (no counterpart in the reference).
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture (including the first call of a shape), tight serving loops,
+ * harnesses that forbid hipMalloc.  Same semantics and options as
+ * t1m_contract, except: A, B and C must be device memory; autotune is
+ * rejected; no plan is cached -- every call rebuilds the contraction's index
+ * maps (one extra small launch) in a static device array of
+ * T1M_NOALLOC_SCRATCH_BYTES per GPU, and a contraction whose maps do not fit
+ * (2 x (M + N + K) offsets of 4 or 8 bytes, plus 24 bytes per batch problem)
+ * returns T1M_ERROR_UNSUPPORTED.  All calls on one GPU share that array:
+ * calls that overlap on one GPU race (serialise them on one stream, or wait
+ * between streams).
+ */
+#define T1M_NOALLOC_SCRATCH_BYTES (8u << 20)
+int t1m_contract_noalloc(const t1m_tensor* a, const char* la, const t1m_tensor* b,
+                         const char* lb, const t1m_tensor* c, const char* lc,
+                         const t1m_options* opt);
+
+/*
  * Batched contraction (no counterpart in the reference): count independent
  * contractions sharing one label triple and dtype, each with its own extents,
  * strides and device pointers (tensor-network sweeps).  create() builds the

@@ -59,7 +59,7 @@ class T1MError(RuntimeError):
 
 
 # Exported symbols of include/t1m_hip.h.
-EXPORTS = ["t1m_options_init", "t1m_contract", "t1m_contract_vjp",
+EXPORTS = ["t1m_options_init", "t1m_contract", "t1m_contract_noalloc", "t1m_contract_vjp",
            "t1m_batch_create", "t1m_batch_launch", "t1m_batch_destroy",
            "t1m_index_bundles", "t1m_index_maps", "t1m_pack_block", "t1m_unpack",
            "t1m_last_error", "t1m_version", "t1m_plan_cache_size",
@@ -85,6 +85,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.t1m_options_init.argtypes = [P(Options)]
     L.t1m_options_init.restype = None
     L.t1m_contract.argtypes = [P(TensorDesc), ctypes.c_char_p] * 3 + [P(Options)]
+    L.t1m_contract_noalloc.argtypes = [P(TensorDesc), ctypes.c_char_p] * 3 + [P(Options)]
     L.t1m_contract_vjp.argtypes = ([P(TensorDesc), ctypes.c_char_p] * 3 +
                                    [P(TensorDesc), P(TensorDesc), P(Options)])
     L.t1m_batch_create.argtypes = ([ctypes.c_int64] + [P(TensorDesc), ctypes.c_char_p] * 3 +
@@ -235,6 +236,28 @@ def contract(a, la: str, b, lb: str, c, lc: str, alpha=1.0, precision="native",
                                lb.encode(), ctypes.byref(dc.desc), lc.encode(),
                                ctypes.byref(o)))
     return c
+
+
+def contract_noalloc(a, la: str, b, lb: str, c, lc: str, alpha=1.0, precision="native",
+                     conj_a: bool = False, conj_b: bool = False, stream=None,
+                     synchronize: bool = True, deterministic: bool = False):
+    """contract() without device allocation (t1m_contract_noalloc): device
+    tensors only, no plan cache (the index maps are rebuilt per call in a
+    static per-GPU array of NOALLOC_SCRATCH_BYTES), no autotuning.  Calls
+    that overlap on one GPU race on that array.  Returns c."""
+    a, ca = _operand(a)
+    b, cb = _operand(b)
+    _output(c, "c")
+    da, db, dc = _Desc(a), _Desc(b), _Desc(c)
+    o = _options(alpha, precision, bool(conj_a) ^ ca, bool(conj_b) ^ cb, stream,
+                 synchronize, False, deterministic)
+    _check(load().t1m_contract_noalloc(ctypes.byref(da.desc), la.encode(),
+                                       ctypes.byref(db.desc), lb.encode(),
+                                       ctypes.byref(dc.desc), lc.encode(), ctypes.byref(o)))
+    return c
+
+
+NOALLOC_SCRATCH_BYTES = 8 << 20     # T1M_NOALLOC_SCRATCH_BYTES
 
 
 def contract_vjp(a, la: str, b, lb: str, grad_c, lc: str, grad_a=None, grad_b=None,

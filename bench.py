@@ -10,9 +10,16 @@ paths' relative Frobenius errors are measured on log-uniform-magnitude inputs
 against a complex128 reference.
 
 Timing: `value` is flop x steps over the host wall time of `steps` calls
-issued back to back between two synchronizations; `roofline.achieved` and
-the kernel_ms figures come from a pass of the same calls just before it,
-each bracketed by HIP events on the launch stream (see timed_region).
+issued back to back between two synchronizations; `roofline.kernel_ms` is
+the GPU time of that same region (HIP events on the launch stream before
+the first and after the last call) per launch, and `roofline.achieved` the
+flops over it; kernel_ms_median / _min come from a pass just before, each
+call bracketed by its own events (see timed_region).
+
+The same line carries the other timed configurations as sub-objects: "c3"
+(complex<double> asb,btc->astc 256x4x256 on FP64 MFMA) and "c5" (512 MPS
+transfer contractions in one persistent graph launch, plus their VJP),
+each with its value, roofline, errors and CPU baseline.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -29,6 +36,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2516.6}   # MI355X dense TFLOP/s
+METRIC = "complex contraction TFLOP/s, % MFMA peak; BF16x3 vs FP32 at rel err <=1e-5"
 
 CONFIGS = {
     "c2": dict(la="abcd", lb="dbef", lc="acef", dtype="complex64",
@@ -82,12 +90,13 @@ def make_inputs(cfg, kind, seed, torch):
 def timed_region(torch, stream, fn, steps, warmup):
     """`warmup` untimed calls, then two passes of `steps` calls each:
     (1) every call bracketed by HIP events on the launch stream, giving each
-    launch's duration (and warming the clocks further); (2) the timed region
-    proper -- synchronize, host clock, `steps` calls back to back,
-    synchronize -- giving the whole-job wall time.  The per-launch events
-    are kept out of (2): each event record puts a marker between two
-    launches that idles the GPU for a few microseconds, which is
-    measurement, not workload."""
+    launch's duration on its own (median / min, and warmer clocks); (2) the
+    timed region proper -- synchronize, host clock and a HIP event on the
+    launch stream, `steps` calls back to back, an event, synchronize.  The
+    two events of (2) give the GPU time of the region, so its average launch
+    duration (`kernel_ms`, the roofline's denominator) comes from the same
+    launches as the wall time and never exceeds it.  Returns (wall seconds,
+    region kernel ms per launch, per-launch ms of pass 1)."""
     for _ in range(max(warmup, 1)):
         fn()
     torch.cuda.synchronize()
@@ -98,25 +107,41 @@ def timed_region(torch, stream, fn, steps, warmup):
         fn()
         e_.record(stream)
     torch.cuda.synchronize()
+    r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
+    r0.record(stream)
     for _ in range(steps):
         fn()
+    r1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return wall, [s_.elapsed_time(e_) for s_, e_ in evs]
+    return wall, r0.elapsed_time(r1) / steps, [s_.elapsed_time(e_) for s_, e_ in evs]
+
+
+def cold_call_ms(t1m, torch, a, b, c, cfg, precision):
+    """One synchronous call with an empty plan cache: label parsing, the
+    device index maps (one launch and a wait) and the contraction -- what
+    the reference pays on every call (benchmark.cpp:68-75 times bundles,
+    scatter vectors and workspace allocation inside each repeat)."""
+    t1m.plan_cache_clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision)
+    return (time.perf_counter() - t0) * 1e3
 
 
 def time_path(t1m, torch, a, b, c, cfg, precision, steps, warmup):
     stream = torch.cuda.current_stream()
-    # the first call autotunes the tile shape
+    cold = cold_call_ms(t1m, torch, a, b, c, cfg, precision)
+    # the next call autotunes the tile shape
     t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision,
                  synchronize=False, autotune=True)
-    wall, kern_ms = timed_region(
+    wall, kern_ms, per_launch = timed_region(
         torch, stream,
         lambda: t1m.contract(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision,
                              synchronize=False), steps, warmup)
     kernel = t1m.plan_kernel(a, cfg["la"], b, cfg["lb"], c, cfg["lc"], precision=precision)
-    return wall, kern_ms, kernel
+    return wall, kern_ms, per_launch, kernel, cold
 
 
 def rel_errors(t1m, torch, cfg, seed):
@@ -151,6 +176,11 @@ def cpu_threads():
         return os.cpu_count() or 1
 
 
+CPU_KERNEL_NOTE = ("BLIS-sized blocks; a plain C micro-kernel built with -ffp-contract=off, "
+                   "so no FMA and no BLIS assembly micro-kernel: the reference's BLIS "
+                   "haswell kernel would be several times faster per core")
+
+
 def cpu_baseline(cfg, seconds_hint=15.0, threads=1):
     """The oracle's five-loop 1M restatement (one thread: the reference is
     serial; or loop 3 over `threads` OpenMP threads) on a bounded slice of
@@ -183,8 +213,8 @@ def cpu_baseline(cfg, seconds_hint=15.0, threads=1):
         cut *= 2
     return {"value": round(fl / dt_s / 1e12, 6), "unit": "TFLOP/s", "cores": threads,
             "kind": "port",
-            "sample": (f"oracle five-loop 1M contraction (BLIS-sized blocks, plain C "
-                       f"micro-kernel), {cfg['la']},{cfg['lb']}->{cfg['lc']} with "
+            "sample": (f"oracle five-loop 1M contraction ({CPU_KERNEL_NOTE}), "
+                       f"{cfg['la']},{cfg['lb']}->{cfg['lc']} with "
                        f"'{lastb}' cut to {cut} of {dims[lastb]} ({fl:.3e} flop, "
                        f"{dt_s:.2f} s, {threads} thread{'s' if threads > 1 else ''})")}
 
@@ -382,7 +412,7 @@ def traffic_bytes(config, kernel):
         return None
 
 
-def bench_c5(args, torch, t1m):
+def c5_result(args, torch, t1m, cpu_seconds=15.0):
     """512 complex<float> MPS transfer contractions 'asb,btc->astc' (physical
     dimension 4, bond dimensions uniform over {16, 32, ..., 256}), one
     persistent graph launch, then the VJP of L = sum |C|^2 (grad_c = 2C):
@@ -406,53 +436,67 @@ def bench_c5(args, torch, t1m):
     GB = [torch.zeros_like(b) for b in B]
     fwd_flop = sum(128 * dl * db * dr for dl, db, dr in dims)
     stream = torch.cuda.current_stream()
+    t1m.plan_cache_clear()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     fwd = t1m.Batch(A, "asb", B, "btc", C, "astc", stream=stream, synchronize=False)
+    build_s = time.perf_counter() - t0
     vja = t1m.Batch(G, "astc", B, "btc", GA, "asb", conj_b=True, stream=stream,
                     synchronize=False)
     vjb = t1m.Batch(A, "asb", G, "astc", GB, "btc", conj_a=True, stream=stream,
                     synchronize=False)
-    build_s = time.perf_counter() - t0
 
-    def timed(fn):
-        wall, kern = timed_region(torch, stream, fn, args.steps, args.warmup)
-        return wall, sum(kern) / len(kern)
-
-    wall_f, kf = timed(lambda: fwd.launch(stream))
-    wall_v, kv = timed(lambda: (vja.launch(stream), vjb.launch(stream)))
-    line = {
-        "metric": "complex contraction TFLOP/s, % MFMA peak; BF16x3 vs FP32 at rel err <=1e-5",
+    wall_f, kf, per_f = timed_region(torch, stream, lambda: fwd.launch(stream), args.steps,
+                                     args.warmup)
+    wall_v, kv, _ = timed_region(torch, stream, lambda: (vja.launch(stream), vjb.launch(stream)),
+                                 args.steps, args.warmup)
+    per_f = sorted(per_f)
+    out = {
         "value": round(fwd_flop * args.steps / wall_f / 1e12, 3), "unit": "TFLOP/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(wall_f / args.steps * 1e3, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
-        "data": "synthetic (seeded bond dims, re/im i.i.d. U[-1,1]; device-resident)",
-        "config": {"workload": "512 complex<float> MPS transfer contractions asb,btc->astc, "
-                               "phys 4, bonds in {16..256}, one persistent graph launch",
-                   "flop_per_step": fwd_flop, "instances": len(dims),
-                   "plan_build_s": round(build_s, 3)},
+        "ms_per_step": round(wall_f / args.steps * 1e3, 4),
+        "workload": "512 complex<float> MPS transfer contractions asb,btc->astc, phys 4, "
+                    "bonds in {16..256} (seed 5), one persistent graph launch",
+        "flop_per_step": fwd_flop, "instances": len(dims),
+        "cold_build_ms": round(build_s * 1e3, 1),
         "roofline": {"bound": "mfma", "achieved": round(fwd_flop / (kf * 1e-3) / 1e12, 3),
                      "peak": PEAK["fp32"], "unit": "TFLOP/s",
                      "frac": round(fwd_flop / (kf * 1e-3) / 1e12 / PEAK["fp32"], 4),
                      "traffic": traffic_bytes("c5", "c64_f32mfma_64x64_batched_fwd"),
-                     "kernel_ms": round(kf, 4)},
+                     "kernel_ms": round(kf, 4), "kernel_ms_median": round(per_f[len(per_f) // 2], 4)},
         "vjp": {"value": round(2 * fwd_flop * args.steps / wall_v / 1e12, 3), "unit": "TFLOP/s",
-                "ms_per_step": round(wall_v / args.steps * 1e3, 4),
+                "ms_per_step": round(wall_v / args.steps * 1e3, 4), "kernel_ms": round(kv, 4),
                 "flop_per_step": 2 * fwd_flop},
     }
     for bt in (fwd, vja, vjb):
         bt.close()
     if not args.no_errors:
-        line["rel_err_vs_fp64"] = c5_errors(torch, t1m, A, B, dims, stream)
+        out["rel_err_vs_fp64"] = c5_errors(torch, t1m, A, B, dims, stream)
     if not args.no_cpu:
-        line["cpu_baseline"] = c5_cpu_baseline(dims)
+        out["cpu_baseline"] = c5_cpu_baseline(dims, seconds_hint=cpu_seconds)
+    return out
+
+
+def bench_c5(args, torch, t1m):
+    r = c5_result(args, torch, t1m)
+    line = {
+        "metric": METRIC, "value": r["value"], "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded bond dims, re/im i.i.d. U[-1,1]; device-resident)",
+        "config": {"workload": r["workload"], "flop_per_step": r["flop_per_step"],
+                   "instances": r["instances"], "parallelism": "single GPU"},
+    }
+    line.update({k: v for k, v in r.items() if k not in ("value", "unit", "ms_per_step",
+                                                          "workload", "flop_per_step",
+                                                          "instances")})
     print(json.dumps(line), flush=True)
 
 
-def c5_errors(torch, t1m, A, B, dims, stream, count=16):
+def c5_errors(torch, t1m, A, B, dims, stream, count=512):
     """Relative Frobenius error of a fresh persistent batch over the first
-    `count` instances against complex128 products of the same operands
-    (permuted to GEMM layout, contiguous: (a s) x b times b x (t c))."""
+    `count` instances (all 512 by default) against complex128 products of
+    the same operands (permuted to GEMM layout, contiguous: (a s) x b times
+    b x (t c))."""
     sel = range(min(count, len(dims)))
     C = [torch.zeros((dims[i][2], 4, 4, dims[i][0]), dtype=torch.complex64,
                      device="cuda").permute(3, 2, 1, 0) for i in sel]
@@ -497,9 +541,9 @@ def c5_cpu_baseline(dims, seconds_hint=15.0):
         if spent > seconds_hint:
             break
     return {"value": round(fl / spent / 1e12, 6), "unit": "TFLOP/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle five-loop 1M contraction (BLIS-sized blocks, plain C "
-                       f"micro-kernel), the first {n} of {len(dims)} instances "
-                       f"asb,btc->astc ({fl:.3e} flop, {spent:.2f} s, 1 thread)")}
+            "sample": (f"oracle five-loop 1M contraction ({CPU_KERNEL_NOTE}), the first "
+                       f"{n} of {len(dims)} instances asb,btc->astc ({fl:.3e} flop, "
+                       f"{spent:.2f} s, 1 thread)")}
 
 
 def bench_maps(args, torch, t1m):
@@ -519,8 +563,7 @@ def bench_maps(args, torch, t1m):
     out = {}
 
     def rate(name, fn, nbytes):
-        _, kern = timed_region(torch, stream, fn, args.steps, args.warmup)
-        ms = sum(kern) / len(kern)
+        _, ms, _ = timed_region(torch, stream, fn, args.steps, args.warmup)
         out[name] = {"GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1), "us": round(ms * 1e3, 2),
                      "bytes": int(nbytes), "frac_of_hbm_peak": round(nbytes / (ms * 1e-3) / 8e12, 4)}
 
@@ -581,6 +624,8 @@ def main():
     ap.add_argument("--no-errors", action="store_true", help="skip the error report")
     ap.add_argument("--no-library", action="store_true",
                     help="skip the vendor GEMM and hipTensor comparisons")
+    ap.add_argument("--only", action="store_true",
+                    help="with the default c2: skip the c3 and c5 sub-objects")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -608,42 +653,66 @@ def main():
     if args.config == "maps":
         bench_maps(args, torch, t1m)
         return
-    cfg = CONFIGS[args.config]
+    line = config_result(args.config, args, torch, t1m, headline=True)
+    if args.config == "c2" and not args.only:
+        # the other timed configurations, so one default run reports them all
+        line["c3"] = config_result("c3", args, torch, t1m, headline=False)
+        line["c5"] = c5_result(args, torch, t1m, cpu_seconds=8.0)
+    print(json.dumps(line), flush=True)
+
+
+def config_result(name, args, torch, t1m, headline):
+    """One single-contraction configuration: FP32 / FP64 MFMA path timed
+    (and BF16x3 for complex<float>), errors against complex128, comparison
+    libraries and the CPU baseline.  `headline` gives the full JSON line
+    (else a sub-object for another configuration)."""
+    cfg = CONFIGS[name]
     fl, (m, n, k) = flops(cfg)
     a, b, c = make_inputs(cfg, "uniform" if cfg["dtype"] == "complex64" else "normal", 2, torch)
 
-    wall, kern, kname = time_path(t1m, torch, a, b, c, cfg, "native", args.steps, args.warmup)
+    wall, kern_ms, per_launch, kname, cold = time_path(t1m, torch, a, b, c, cfg, "native",
+                                                       args.steps, args.warmup)
     ms_step = wall / args.steps * 1e3
     value = fl * args.steps / wall / 1e12
-    kern_avg = sum(kern) / len(kern)
-    kern_sorted = sorted(kern)
-    achieved = fl / (kern_avg * 1e-3) / 1e12
+    per_launch = sorted(per_launch)
+    achieved = fl / (kern_ms * 1e-3) / 1e12
     peak = PEAK[cfg["peak"]]
-    line = {
-        "metric": "complex contraction TFLOP/s, % MFMA peak; BF16x3 vs FP32 at rel err <=1e-5",
-        "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32" if cfg["dtype"] == "complex64" else "fp64",
-        "data": "synthetic (seeded, re/im i.i.d. U[-1,1]; device-resident)",
-        "config": {"workload": cfg["workload"], "labels": f"{cfg['la']},{cfg['lb']}->{cfg['lc']}",
-                   "m": m, "n": n, "k": k, "flop_per_step": fl, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                     "traffic": traffic_bytes(args.config, kname),
-                     "kernel_ms": round(kern_avg, 4), "kernel": kname,
-                     "kernel_ms_median": round(kern_sorted[len(kern) // 2], 4),
-                     "kernel_ms_min": round(kern_sorted[0], 4)},
-    }
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+            "traffic": traffic_bytes(name, kname),
+            "kernel_ms": round(kern_ms, 4), "kernel": kname,
+            "kernel_ms_median": round(per_launch[len(per_launch) // 2], 4),
+            "kernel_ms_min": round(per_launch[0], 4)}
+    dtype = "fp32" if cfg["dtype"] == "complex64" else "fp64"
+    if headline:
+        line = {
+            "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype,
+            "data": "synthetic (seeded, re/im i.i.d. U[-1,1]; device-resident)",
+            "config": {"workload": cfg["workload"],
+                       "labels": f"{cfg['la']},{cfg['lb']}->{cfg['lc']}",
+                       "m": m, "n": n, "k": k, "flop_per_step": fl,
+                       "parallelism": "single GPU"},
+            "roofline": roof,
+        }
+    else:
+        line = {"value": round(value, 3), "unit": "TFLOP/s", "ms_per_step": round(ms_step, 4),
+                "dtype": dtype, "workload": cfg["workload"], "flop_per_step": fl,
+                "roofline": roof}
+    line["cold_call_ms"] = round(cold, 3)
     if cfg["dtype"] == "complex64":
         c2 = torch.zeros_like(c)
-        wall_b, kern_b, kname_b = time_path(t1m, torch, a, b, c2, cfg, "bf16x3", args.steps,
-                                            args.warmup)
-        kb = sum(kern_b) / len(kern_b)
+        wall_b, kb, _, kname_b, _ = time_path(t1m, torch, a, b, c2, cfg, "bf16x3", args.steps,
+                                              args.warmup)
         line["bf16x3"] = {"value": round(fl * args.steps / wall_b / 1e12, 3), "unit": "TFLOP/s",
                           "kernel_ms": round(kb, 4), "kernel": kname_b,
-                          "speedup_vs_fp32": round(kern_avg / kb, 3),
-                          "frac_of_fp32_peak": round(fl / (kb * 1e-3) / 1e12 / peak, 4)}
+                          "speedup_vs_fp32": round(kern_ms / kb, 3),
+                          "frac_of_fp32_peak": round(fl / (kb * 1e-3) / 1e12 / peak, 4),
+                          "frac_of_bf16_peak_3_products":
+                              round(3 * fl / (kb * 1e-3) / 1e12 / PEAK["bf16"], 4),
+                          "traffic": traffic_bytes(name, kname_b)}
     if not args.no_errors:
         errs = rel_errors(t1m, torch, cfg, 4)
         line["rel_err_vs_fp64"] = {p: float(f"{e:.3e}") for p, e in errs.items()}
@@ -651,12 +720,13 @@ def main():
         line["library_gemm"] = library_gemm(cfg, torch, a, b, torch.zeros_like(c), args.steps)
         line["hiptensor"] = hiptensor_contract(cfg, torch, t1m, a, b, c, args.steps)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(cfg)
+        line["cpu_baseline"] = cpu_baseline(cfg, seconds_hint=15.0 if headline else 8.0)
         nt = cpu_threads()
-        if nt > 1:
+        if nt > 1 and headline:
             line["cpu_baseline_all_cores"] = cpu_baseline(cfg, seconds_hint=5.0, threads=nt)
-        line["cpu_torch_einsum"] = cpu_einsum(cfg, torch)
-    print(json.dumps(line), flush=True)
+        if headline:
+            line["cpu_torch_einsum"] = cpu_einsum(cfg, torch)
+    return line
 
 
 if __name__ == "__main__":

@@ -17,7 +17,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libt1m_oracle.so")
+# T1M_ORACLE_LIB selects another build (the sanitizer build).
+_LIB_PATH = os.environ.get("T1M_ORACLE_LIB") or os.path.join(_HERE, "_build", "libt1m_oracle.so")
 MAX_RANK = 32
 
 DTYPES = {np.dtype(np.float32): 0, np.dtype(np.float64): 1,

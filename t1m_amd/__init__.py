@@ -14,7 +14,9 @@ import os
 from typing import Optional, Sequence
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libt1m_hip.so")
+# T1M_LIB selects another build of the library (the sanitizer build of
+# tests/run_sanitized.sh); by default the in-tree libt1m_hip.so.
+LIB_PATH = os.environ.get("T1M_LIB") or os.path.join(_HERE, "libt1m_hip.so")
 MAX_RANK = 16
 
 FLOAT32, FLOAT64, COMPLEX64, COMPLEX128 = 0, 1, 2, 3

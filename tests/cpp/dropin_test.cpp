@@ -207,10 +207,10 @@ int main() {
                               {4, 3, 5, 2, 3}, {6, 3}, {2, 5, 6, 3, 4}, 1e-12);
   vjp_case<std::complex<float>>(1e-5);
   vjp_case<std::complex<double>>(1e-12);
-  suite<float>(12);
-  suite<double>(12);
-  suite<std::complex<float>>(12);
-  suite<std::complex<double>>(12);
+  suite<float>(20);   // d = 2..20 as test_contraction.cpp:49,56
+  suite<double>(20);
+  suite<std::complex<float>>(20);
+  suite<std::complex<double>>(20);
   // Errors the reference leaves undefined are reported.
   try {
     std::vector<std::complex<float>> x(8);

@@ -21,23 +21,19 @@
 //
 // LDS holds one 32-bit word per complex element and plane (hi, lo), rows of
 // m (for A) / n (for B) with the k index contiguous, so a lane's fragment is
-// one 16-byte read per plane.  On the 144 x 144 tiles A's rows are stored
-// already expanded (real rows 2i + s), so the fragment reads need no
-// per-reader perm / xor.  Operands gathered along m / n are staged in quads
-// of four consecutive k (one 16-byte LDS store per plane and row).
+// one 16-byte read per plane.  Operands gathered along m / n are staged in
+// quads of four consecutive k (one 16-byte LDS store per plane and row).
+// That is the 64 x 64 family below; the 144 x 144 tiles use the row-pair
+// form further down.
 #include "internal.h"
 #include "kernel_common.h"
 
 #include <type_traits>
 
-// Layout choices, measured on complex<float> 48^4 contractions in the four
-// gather-direction combinations (timing variants built from switches that
-// no longer exist): expanded A (+11 % on 144 x 144 tiles, a loss on 64 x 64
-// where the larger LDS tile costs occupancy), quad staging of A (+5..15 %)
-// and of B (64 x 64 tiles only).  Also measured and dropped: LDS writes
-// inside the MFMA stream instead of after it (no gain), and unpadded rows
-// with XOR-swizzled 16-byte chunks (+3 % on one direction combination, -5 %
-// and -10 % on two others).
+// Layout choices for the 64 x 64 tiles, measured on complex<float> 48^4
+// contractions in the four gather-direction combinations (timing variants
+// built from switches that no longer exist): A unexpanded (expanded rows
+// cost occupancy), quad staging of A (+5..15 %) and of B.
 
 namespace t1m_hip {
 
@@ -106,11 +102,16 @@ struct QuadGather {
         koff[u][t] = kmap[kg < K ? kg : K - 1];
       }
   }
-  __device__ void load_data(const float2* __restrict__ base) {
+  __device__ void load_data(const float2* __restrict__ base) { load_data_to(base, val); }
+  __device__ void load_data_to(const float2* __restrict__ base, float2 (&out)[E][4]) const {
 #pragma unroll
     for (int u = 0; u < E; ++u)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) val[u][t] = base[xoff[u] + koff[u][t]];
+      for (int t = 0; t < 4; ++t) out[u][t] = base[xoff[u] + koff[u][t]];
+  }
+  // Whether entry (u, t) of a stage starting at p0 lies inside the problem.
+  __device__ bool inside(unsigned tid, int p0, int K, int u, int t) const {
+    return ((xok >> u) & 1u) && p0 + 4 * kq(elem(tid, u)) + t < K;
   }
   __device__ void finish_one(unsigned tid, int p0, int K, float sign, int u) {
 #pragma unroll
@@ -127,6 +128,10 @@ struct QuadGather {
   }
 };
 
+template <typename G> struct IsQuad : std::false_type {};
+template <typename Idx, int BX, int BK, int NT>
+struct IsQuad<QuadGather<Idx, BX, BK, NT>> : std::true_type {};
+
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool AF, bool BF>
 __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm, int tn, int kbeg, int kend) {
   constexpr int NT = 64 * WGM * WGN;
@@ -138,10 +143,9 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   constexpr int XB = XT ? (MB + WGN - 1) / WGN : 0;
   constexpr int XB_ALLOC = XT ? XB : 1;
   constexpr int PITCH = BK + 4;   // words per LDS row (padded: conflict-light reads)
-  constexpr bool AEXP = BM >= 144;
-  constexpr int AROWS = AEXP ? 2 * BM : BM;
-  __shared__ __attribute__((aligned(16))) uint32_t sAh[2][AROWS][PITCH];
-  __shared__ __attribute__((aligned(16))) uint32_t sAl[2][AROWS][PITCH];
+  static_assert(BM < 144, "144 x 144 tiles use the row-pair kernel");
+  __shared__ __attribute__((aligned(16))) uint32_t sAh[2][BM][PITCH];
+  __shared__ __attribute__((aligned(16))) uint32_t sAl[2][BM][PITCH];
   __shared__ __attribute__((aligned(16))) uint32_t sBh[2][BN][PITCH];
   __shared__ __attribute__((aligned(16))) uint32_t sBl[2][BN][PITCH];
 
@@ -157,9 +161,7 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   const unsigned utid = unsigned(tid);
   // Gathers along x store quads (16-byte writes); gathers along k store
   // single words, consecutive lanes filling consecutive words of a row.
-  // (Measured: quads pay for A at both tile sizes and for B only on the
-  // 64 x 64 tiles.)
-  constexpr bool QA = AF, QB = BF && BM < 144;
+  constexpr bool QA = AF, QB = BF;
   using GA = std::conditional_t<QA, QuadGather<Idx, BM, BK, NT>,
                                 TileGather<float2, Idx, BM, BK, NT, AF>>;
   using GB = std::conditional_t<QB, QuadGather<Idx, BN, BK, NT>,
@@ -171,9 +173,6 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   const float sa = p.conj_a ? -1.f : 1.f;
   const float sb = p.conj_b ? -1.f : 1.f;
 
-  // Expanded A rows: 2x holds (re, -im), 2x + 1 holds (im, re).
-  auto exp0 = [](uint32_t w) { return w ^ 0x80000000u; };
-  auto exp1 = [](uint32_t w) { return __builtin_amdgcn_alignbit(w, w, 16); };
   // Stage one gathered unit (an element, or a quad of four k) of A / B into
   // LDS buffer buf: zero / conjugate, split into hi and lo words, store.
   auto write_a = [&](int buf, int p0, int u) {
@@ -190,36 +189,16 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
       }
       if (GA::valid(utid, u)) {
         const int x = GA::xc(e), kw = 4 * GA::kq(e);
-        if constexpr (AEXP) {
-          u32x4 h0, h1, l0, l1;
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            h0[t] = exp0(hi[t]); h1[t] = exp1(hi[t]);
-            l0[t] = exp0(lo[t]); l1[t] = exp1(lo[t]);
-          }
-          *reinterpret_cast<u32x4*>(&sAh[buf][2 * x][kw]) = h0;
-          *reinterpret_cast<u32x4*>(&sAh[buf][2 * x + 1][kw]) = h1;
-          *reinterpret_cast<u32x4*>(&sAl[buf][2 * x][kw]) = l0;
-          *reinterpret_cast<u32x4*>(&sAl[buf][2 * x + 1][kw]) = l1;
-        } else {
-          *reinterpret_cast<u32x4*>(&sAh[buf][x][kw]) = hi;
-          *reinterpret_cast<u32x4*>(&sAl[buf][x][kw]) = lo;
-        }
+        *reinterpret_cast<u32x4*>(&sAh[buf][x][kw]) = hi;
+        *reinterpret_cast<u32x4*>(&sAl[buf][x][kw]) = lo;
       }
     } else {
       uint32_t hi, lo;
       split_pair(ga.val[u].x, ga.val[u].y, hi, lo);
       if (GA::valid(utid, u)) {
         const int x = GA::xc(e), k = GA::kc(e);
-        if constexpr (AEXP) {
-          sAh[buf][2 * x][k] = exp0(hi);
-          sAh[buf][2 * x + 1][k] = exp1(hi);
-          sAl[buf][2 * x][k] = exp0(lo);
-          sAl[buf][2 * x + 1][k] = exp1(lo);
-        } else {
-          sAh[buf][x][k] = hi;
-          sAl[buf][x][k] = lo;
-        }
+        sAh[buf][x][k] = hi;
+        sAl[buf][x][k] = lo;
       }
     }
   };
@@ -294,10 +273,7 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   // Word column of this lane's 4 complex k in the rows it reads.
   const int colA = 4 * g, colB = 4 * g;
   auto ldsA = [&](const uint32_t (*plane)[PITCH], int row) {
-    if constexpr (AEXP)
-      return *reinterpret_cast<const u32x4*>(&plane[2 * row + (odd ? 1 : 0)][colA]);
-    else
-      return expand(*reinterpret_cast<const u32x4*>(&plane[row][colA]));
+    return expand(*reinterpret_cast<const u32x4*>(&plane[row][colA]));
   };
   auto ldsB = [&](const uint32_t (*plane)[PITCH], int row) {
     return *reinterpret_cast<const u32x4*>(&plane[row][colB]);
@@ -307,10 +283,9 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
   // block.  Three products per tile: lo*hi and hi*lo first, then hi*hi.
   auto compute = [&](int buf) {
     static_assert(BK == 16, "one MFMA k-chunk per stage");
-    // Raised wave priority over the MFMA stream: +4..7 % on 64 x 64 tiles,
-    // -7..-15 % on 144 x 144 (measured on four 48^4 layouts).
-    constexpr bool PRIO = BM < 144;
-    if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
+    // Raised wave priority over the MFMA stream: +4..7 % on 64 x 64 tiles
+    // (measured on four 48^4 layouts).
+    __builtin_amdgcn_s_setprio(1);
     u32x4 bh[NBW], bl[NBW];
 #pragma unroll
     for (int nb = 0; nb < NBW; ++nb) {
@@ -340,7 +315,7 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
         accx[x] = mfma_bf16(xbh, ah, accx[x]);
       }
     }
-    if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
+    __builtin_amdgcn_s_setprio(0);
   };
 
   const int nst = (K - kbeg + BK - 1) / BK;
@@ -435,6 +410,340 @@ k_contract_bf16x3_batched(const KArgs<Idx>* __restrict__ inst,
   }
 }
 
+// ---------------------------------------------------------------------------
+// 144 x 144 tiles in the row-pair form.
+//
+// MFMA columns are 16 complex rows i of C (not 16 real rows of C_r), and
+// the 1M expansion sits on B: LDS holds A's packed (re, im) words as they
+// are and, for every B element, the two words that pair with them,
+// (re, -im) for the real part of C and (im, re) for the imaginary part
+// (sum_p a_re b_re - a_im b_im and a_re b_im + a_im b_re).  Conjugating A
+// or B only flips sign bits of those two words (two xor masks), and the
+// words are formed once, when B is staged, so the MFMA operands come
+// straight from LDS.  A lane's outputs are whole complex elements, C[i][j]
+// for i = l16 of a 16-row unit and j = 4 g + t of a 16-column unit.
+//
+// The tile is 9 x 9 units of 16 complex rows x 16 columns.  Each wave owns
+// a rectangle of units (its A row fragments and B column fragments per
+// k-chunk, reused across the rectangle), laid out so that the two waves of
+// each SIMD (waves w and w + 4) hold 20 or 21 units between them.  LDS rows
+// are 16 words (16 complex k) without padding; the 16-byte chunk c of row x
+// sits at chunk c ^ ((x >> 2) & 3): fragment reads (16 rows, one chunk) and
+// the quad and word writes of the gathers all run at the contiguous rate
+// (measured with an LDS pattern probe on gfx950).
+//
+// Out-of-range rows, columns and k (edge tiles, the last stage) are zeroed
+// when staged; tiles and stages that are wholly inside skip those selects.
+
+struct WaveRect { int r0, nr, c0, nc; };
+constexpr WaveRect kRect144[8] = {
+    {0, 5, 0, 2}, {0, 5, 4, 2}, {5, 2, 0, 5}, {5, 4, 5, 3},
+    {0, 5, 2, 2}, {0, 5, 6, 2}, {7, 2, 0, 5}, {0, 9, 8, 1}};
+
+constexpr int kP144 = 144;             // tile rows / columns
+constexpr int kPW = 16;                // words per LDS row (16 complex k)
+constexpr int kPPlane = kP144 * kPW;   // words per plane
+constexpr int kPPlanes = 6;            // A hi, A lo, B re-hi, B re-lo, B im-hi, B im-lo
+
+// Word offset of word w of LDS row x (chunk swizzle).
+__device__ inline int pword(int x, int w) {
+  return x * kPW + ((((w >> 2) ^ (x >> 2)) & 3) << 2) + (w & 3);
+}
+
+template <typename Idx, bool AF, bool BF, int NR, int NC>
+__device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* __restrict__ smem,
+                                                 int tm, int tn, int kbeg, int kend, int r0,
+                                                 int c0) {
+  constexpr int BM = kP144, BN = kP144, BK = 16, NT = 512;
+  // smem: [buf][plane][row][16 words]
+  auto plane = [&](int buf, int pl) { return smem + (buf * kPPlanes + pl) * kPPlane; };
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
+  const int64_t M = p.M, N = p.N;
+  const int K = kend;
+  const float2* __restrict__ A = static_cast<const float2*>(p.A);
+  const float2* __restrict__ B = static_cast<const float2*>(p.B);
+  const unsigned utid = unsigned(tid);
+
+  using GA = std::conditional_t<AF, QuadGather<Idx, BM, BK, NT>,
+                                TileGather<float2, Idx, BM, BK, NT, false>>;
+  using GB = std::conditional_t<BF, QuadGather<Idx, BN, BK, NT>,
+                                TileGather<float2, Idx, BN, BK, NT, false>>;
+  GA ga;
+  GB gb;
+  ga.init(utid, i0, M, p.rsA);
+  gb.init(utid, j0, N, p.csB);
+  const bool a_inside = i0 + BM <= M, b_inside = j0 + BN <= N;
+  // B's words: re-form = w ^ mre, im-form = swap(w) ^ mim (see above; conj
+  // of A and of B flip the signs of the paired terms).
+  const uint32_t mre = (p.conj_a != 0) == (p.conj_b != 0) ? 0x80000000u : 0u;
+  const uint32_t mim = (p.conj_b ? 0x00008000u : 0u) | (p.conj_a ? 0x80000000u : 0u);
+
+  // Stage slot u of a gather (values v) into LDS: zero out-of-range entries
+  // (edge tiles and the last stage only), split into hi / lo, store (A raw,
+  // B as its two forms).
+  auto put_quad = [&](const auto& gt, bool is_b, int buf, int p0, int u, bool edge,
+                      const float2 (&v)[4]) {
+    using G = std::remove_cv_t<std::remove_reference_t<decltype(gt)>>;
+    const unsigned e = G::elem(utid, u);
+    u32x4 hi, lo;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      float re = v[t].x, im = v[t].y;
+      if (edge && !gt.inside(utid, p0, K, u, t)) re = im = 0.f;
+      uint32_t h, l;
+      split_pair(re, im, h, l);
+      hi[t] = h;
+      lo[t] = l;
+    }
+    if (G::valid(utid, u)) {
+      const int w = pword(G::xc(e), 4 * G::kq(e));
+      if (is_b) {
+        u32x4 rh, rl, ih, il;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          rh[t] = hi[t] ^ mre;
+          rl[t] = lo[t] ^ mre;
+          ih[t] = __builtin_amdgcn_alignbit(hi[t], hi[t], 16) ^ mim;
+          il[t] = __builtin_amdgcn_alignbit(lo[t], lo[t], 16) ^ mim;
+        }
+        *reinterpret_cast<u32x4*>(plane(buf, 2) + w) = rh;
+        *reinterpret_cast<u32x4*>(plane(buf, 3) + w) = rl;
+        *reinterpret_cast<u32x4*>(plane(buf, 4) + w) = ih;
+        *reinterpret_cast<u32x4*>(plane(buf, 5) + w) = il;
+      } else {
+        *reinterpret_cast<u32x4*>(plane(buf, 0) + w) = hi;
+        *reinterpret_cast<u32x4*>(plane(buf, 1) + w) = lo;
+      }
+    }
+  };
+  auto put_word = [&](const auto& gt, bool is_b, int buf, int p0, int u, bool edge, float2 v) {
+    using G = std::remove_cv_t<std::remove_reference_t<decltype(gt)>>;
+    const unsigned e = G::elem(utid, u);
+    if (edge) v = gt.finished(utid, p0, K, 1.f, u, v);
+    uint32_t hi, lo;
+    split_pair(v.x, v.y, hi, lo);
+    if (G::valid(utid, u)) {
+      const int w = pword(G::xc(e), G::kc(e));
+      if (is_b) {
+        plane(buf, 2)[w] = hi ^ mre;
+        plane(buf, 3)[w] = lo ^ mre;
+        plane(buf, 4)[w] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ mim;
+        plane(buf, 5)[w] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ mim;
+      } else {
+        plane(buf, 0)[w] = hi;
+        plane(buf, 1)[w] = lo;
+      }
+    }
+  };
+  // Gathered values of one stage, between their request and their staging.
+  using VA = std::conditional_t<AF, float2[GA::E][4], float2[GA::E]>;
+  using VB = std::conditional_t<BF, float2[GB::E][4], float2[GB::E]>;
+  VA va;
+  VB vb;
+  auto load_stage = [&] {
+    ga.load_data_to(A, va);
+    gb.load_data_to(B, vb);
+  };
+  auto swrite = [&](int buf, int p0) {
+    const bool kin = p0 + BK <= K;
+#pragma unroll
+    for (int u = 0; u < GA::E; ++u) {
+      if constexpr (AF) put_quad(ga, false, buf, p0, u, !(a_inside && kin), va[u]);
+      else put_word(ga, false, buf, p0, u, !(a_inside && kin), va[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < GB::E; ++u) {
+      if constexpr (BF) put_quad(gb, true, buf, p0, u, !(b_inside && kin), vb[u]);
+      else put_word(gb, true, buf, p0, u, !(b_inside && kin), vb[u]);
+    }
+  };
+
+  f32x4 acc[NR][NC][2];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      acc[r][c][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+      acc[r][c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+
+  const int g = lane >> 4;
+  const int l16 = lane & 15;
+  auto frag = [&](const uint32_t* pl, int row) {
+    return *reinterpret_cast<const u32x4*>(pl + pword(row, 4 * g));
+  };
+  // Three products per unit and part: lo*hi, hi*lo, hi*hi.
+  auto unit = [&](int r, int c, const u32x4& ah, const u32x4& al, const u32x4& rh,
+                  const u32x4& rl, const u32x4& ih, const u32x4& il) {
+    acc[r][c][0] = mfma_bf16(rl, ah, acc[r][c][0]);
+    acc[r][c][1] = mfma_bf16(il, ah, acc[r][c][1]);
+    acc[r][c][0] = mfma_bf16(rh, al, acc[r][c][0]);
+    acc[r][c][1] = mfma_bf16(ih, al, acc[r][c][1]);
+    acc[r][c][0] = mfma_bf16(rh, ah, acc[r][c][0]);
+    acc[r][c][1] = mfma_bf16(ih, ah, acc[r][c][1]);
+  };
+  // One k-chunk: the fragments of one dimension stay in registers -- B's
+  // (four planes, 16 registers a column) when they take no more registers
+  // than A's (two planes, 8 a row).
+  constexpr bool HOLD_B = 2 * NC <= NR;
+  constexpr int NS = HOLD_B ? NR : NC;     // streamed
+  constexpr int NH = HOLD_B ? NC : NR;     // held
+  auto compute = [&](int buf) {
+    u32x4 h0[NH], h1[NH], h2[NH], h3[NH];
+#pragma unroll
+    for (int x = 0; x < NH; ++x) {
+      if constexpr (HOLD_B) {
+        const int row = (c0 + x) * 16 + l16;
+        h0[x] = frag(plane(buf, 2), row);
+        h1[x] = frag(plane(buf, 3), row);
+        h2[x] = frag(plane(buf, 4), row);
+        h3[x] = frag(plane(buf, 5), row);
+      } else {
+        const int row = (r0 + x) * 16 + l16;
+        h0[x] = frag(plane(buf, 0), row);
+        h1[x] = frag(plane(buf, 1), row);
+      }
+    }
+#pragma unroll
+    for (int y = 0; y < NS; ++y) {
+      if constexpr (HOLD_B) {
+        const int row = (r0 + y) * 16 + l16;
+        const u32x4 ah = frag(plane(buf, 0), row), al = frag(plane(buf, 1), row);
+#pragma unroll
+        for (int x = 0; x < NH; ++x) unit(y, x, ah, al, h0[x], h1[x], h2[x], h3[x]);
+      } else {
+        const int row = (c0 + y) * 16 + l16;
+        const u32x4 rh = frag(plane(buf, 2), row), rl = frag(plane(buf, 3), row);
+        const u32x4 ih = frag(plane(buf, 4), row), il = frag(plane(buf, 5), row);
+#pragma unroll
+        for (int x = 0; x < NH; ++x) unit(x, y, h0[x], h1[x], rh, rl, ih, il);
+      }
+    }
+  };
+
+  // Pipeline (LDS buffers alternate by stage): stage s + 1 is requested at
+  // the top of stage s and staged after its products.  Requests past the
+  // last stage read clamped, valid offsets.  (Measured without gain: a
+  // second register set requesting two stages ahead (spills), staging
+  // between the halves of the products, and skewing the two waves of a
+  // SIMD so that one stages while the other multiplies.)
+  const int nst = (K - kbeg + BK - 1) / BK;
+  ga.load_k(utid, kbeg, K, p.csA);
+  gb.load_k(utid, kbeg, K, p.rsB);
+  load_stage();
+  ga.load_k(utid, kbeg + BK, K, p.csA);
+  gb.load_k(utid, kbeg + BK, K, p.rsB);
+  swrite(0, kbeg);
+  __syncthreads();
+  for (int s = 0; s + 1 < nst; ++s) {
+    load_stage();
+    ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
+    gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
+    compute(s & 1);
+    swrite((s + 1) & 1, kbeg + (s + 1) * BK);
+    __syncthreads();
+  }
+  compute((nst - 1) & 1);
+
+  // Epilogue: lane holds C[i][j] (re, im) for i = unit row + l16 and
+  // j = unit column + 4 g + t.  C is read a row unit at a time (all of its
+  // columns) before any of it is written back.
+  float2* __restrict__ Cc = static_cast<float2*>(p.C);
+  const float ar = float(p.alpha_re), ai = float(p.alpha_im);
+  Idx cofs[NC][4];
+  bool cok[NC][4];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t j = j0 + (c0 + c) * 16 + 4 * g + t;
+      cok[c][t] = j < N;
+      cofs[c][t] = cok[c][t] ? p.csC[j] : Idx(0);
+    }
+  const bool atomic = p.atomic_c != 0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int64_t i = i0 + (r0 + r) * 16 + l16;
+    const bool rok = i < M;
+    const Idx ro = rok ? p.rsC[i] : Idx(0);
+    auto val = [&](int c, int t) {
+      const float re = acc[r][c][0][t], im = acc[r][c][1][t];
+      return float2{ar * re - ai * im, ar * im + ai * re};
+    };
+    if (atomic) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (rok && cok[c][t]) {
+            const float2 v = val(c, t);
+            float* d = reinterpret_cast<float*>(Cc + (ro + cofs[c][t]));
+            unsafeAtomicAdd(d, v.x);
+            unsafeAtomicAdd(d + 1, v.y);
+          }
+    } else {
+      float2 cv[NC][4];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) cv[c][t] = Cc[ro + cofs[c][t]];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (rok && cok[c][t]) {
+            const float2 v = val(c, t);
+            Cc[ro + cofs[c][t]] = float2{cv[c][t].x + v.x, cv[c][t].y + v.y};
+          }
+    }
+  }
+}
+
+template <typename Idx, bool AF, bool BF>
+__global__ void __launch_bounds__(512) k_contract_bf16x3_pair(KArgs<Idx> p) {
+  __shared__ __attribute__((aligned(16))) uint32_t smem[2 * kPPlanes * kPPlane];
+  int tm, tn, kbeg, kend;
+  tile_and_krange(p, tm, tn, kbeg, kend);
+  const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  const int r0 = kRect144[wid & 7].r0, c0 = kRect144[wid & 7].c0;
+  switch (wid) {
+    case 2:
+    case 6:
+      bf16x3_pair_tile<Idx, AF, BF, 2, 5>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      break;
+    case 3:
+      bf16x3_pair_tile<Idx, AF, BF, 4, 3>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      break;
+    case 7:
+      bf16x3_pair_tile<Idx, AF, BF, 9, 1>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      break;
+    default:
+      bf16x3_pair_tile<Idx, AF, BF, 5, 2>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      break;
+  }
+}
+
+template <typename Idx>
+hipError_t launch_pair144(const GemmProblem& g, hipStream_t s) {
+  const KArgs<Idx> a = make_args<Idx>(g, kP144, kP144, 16);
+  const int64_t per = int64_t(a.tiles_m) * a.tiles_n * a.ksplit;
+  const dim3 block(512);
+  const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
+  return launch_batches(a, per, 512, [&](const KArgs<Idx>& q, unsigned grid) {
+    if (af && bf)
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, true, true>), grid, block, 0, s, q);
+    else if (af)
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, true, false>), grid, block, 0, s, q);
+    else if (bf)
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, false, true>), grid, block, 0, s, q);
+    else
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, false, false>), grid, block, 0, s, q);
+  });
+}
+
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN>
 hipError_t launch_t(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, BM, BN, BK);
@@ -455,7 +764,7 @@ hipError_t launch_t(const GemmProblem& g, hipStream_t s) {
 
 template <typename Idx>
 hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
-  if (kernel == K_C64_BF16X3_144) return launch_t<Idx, 144, 144, 16, 2, 4>(g, s);
+  if (kernel == K_C64_BF16X3_144) return launch_pair144<Idx>(g, s);
   return launch_t<Idx, 64, 64, 16, 2, 2>(g, s);
 }
 

@@ -25,7 +25,8 @@ KERNELS = {"simple": 0, "c64_f32mfma_144x144": 1, "c64_f32mfma_64x64": 2,
            "c128_f64mfma_64x64": 3, "c128_f64mfma_32x32": 4,
            "c64_bf16x3_144x144": 5, "c64_bf16x3_64x64": 6, "f32mfma_128x128": 7,
            "f32mfma_64x64": 8, "f64mfma_64x64": 9, "f64mfma_32x32": 10,
-           "c64_f32mfma_128x16": 11, "c128_f64mfma_128x16": 12}
+           "c64_f32mfma_128x16": 11, "c128_f64mfma_128x16": 12,
+           "c128_f64mfma_64x64w16": 13}
 
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _u64p = ctypes.POINTER(ctypes.c_uint64)

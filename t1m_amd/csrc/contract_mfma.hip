@@ -537,6 +537,10 @@ hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
     // 4 x 2 waves (16 x 32 complex per wave, 2 waves per SIMD): +23 % over
     // 2 x 2 at 1024 x 1024 x 256, +3 % at 2304^3.
     case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16, 4, 2>(g, s);
+    // 8 x 2 waves (4 per SIMD, 8 x 32 complex per wave): +2..3 % at
+    // 1024 x 1024 x 256, -3..-10 % on the reference suite's other
+    // complex<double> contractions -- an autotuning candidate only.
+    case K_C128_F64_64W: return launch_1m<double, Idx, 64, 64, 16, 8, 2>(g, s);
     case K_C128_F64_32: return launch_1m<double, Idx, 32, 32, 16, 2, 2>(g, s);
     // Thin C (one 16-column block): 4 waves stacked along m.
     case K_C64_F32_THIN: return launch_1m<float, Idx, 128, 16, 16, 4, 1>(g, s);
@@ -559,6 +563,7 @@ const KernelInfo kInfo[K_COUNT] = {
     {"f64mfma_32x32", 32, 32},
     {"c64_f32mfma_128x16", 128, 16},
     {"c128_f64mfma_128x16", 128, 16},
+    {"c128_f64mfma_64x64w16", 64, 64},
 };
 
 }  // namespace
@@ -610,7 +615,8 @@ auto with_1m_config(int32_t kernel, F&& f) {
     // Stages of 8 k: half the LDS and 160 registers, 3 workgroups per CU
     // instead of 2 (+2 % on 512 MPS contractions; items are short in k).
     case K_C64_F32_64: return f(Batched1M<float, Idx, 64, 64, 8, 2, 2>());
-    case K_C128_F64_64: return f(Batched1M<double, Idx, 64, 64, 16, 2, 2>());
+    case K_C128_F64_64:
+    case K_C128_F64_64W: return f(Batched1M<double, Idx, 64, 64, 16, 2, 2>());
     default: return f(Batched1M<double, Idx, 32, 32, 16, 2, 2>());
   }
 }

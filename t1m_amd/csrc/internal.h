@@ -86,6 +86,7 @@ enum KernelId : int32_t {
   K_F64_32 = 10,       // double, FP64 MFMA, 32x32 tiles
   K_C64_F32_THIN = 11, // complex<float>, 1M on FP32 MFMA, 128x16 tiles (N <= 16)
   K_C128_F64_THIN = 12,// complex<double>, 1M on FP64 MFMA, 128x16 tiles (N <= 16)
+  K_C128_F64_64W = 13, // complex<double>, 1M on FP64 MFMA, 64x64 tiles on 16 waves
   K_COUNT
 };
 

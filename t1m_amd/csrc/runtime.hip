@@ -195,7 +195,7 @@ std::vector<int> kernel_candidates(int dtype, int precision) {
       if (precision == T1M_PRECISION_BF16X3) return {K_C64_BF16X3_144, K_C64_BF16X3_64};
       return {K_C64_F32_144, K_C64_F32_64, K_C64_F32_THIN};
     case T1M_COMPLEX128:
-      return {K_C128_F64_64, K_C128_F64_32, K_C128_F64_THIN};
+      return {K_C128_F64_64, K_C128_F64_64W, K_C128_F64_32, K_C128_F64_THIN};
     case T1M_FLOAT32:
       return {K_F32_128, K_F32_64};
     case T1M_FLOAT64:

@@ -213,6 +213,7 @@ KERNELS_C64 = [("c64_f32mfma_144x144", "native", "fp32"), ("c64_f32mfma_64x64", 
                ("simple", "native", "simple32")]
 KERNELS_C128 = [("c128_f64mfma_64x64", "native", "fp64"), ("c128_f64mfma_32x32", "native", "fp64"),
                 ("c128_f64mfma_128x16", "native", "fp64"),
+                ("c128_f64mfma_64x64w16", "native", "fp64"),
                 ("simple", "native", "fp64")]
 
 

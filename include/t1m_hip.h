@@ -142,7 +142,8 @@ int t1m_contract_noalloc(const t1m_tensor* a, const char* la, const t1m_tensor* 
  * contractions sharing one label triple and dtype, each with its own extents,
  * strides and device pointers (tensor-network sweeps).  create() builds the
  * index maps and an LPT-ordered work list on the device and captures the
- * launch in a HIP graph; launch() replays it.  All data must be device memory.
+ * launch in a HIP graph; launch() replays it.  All data must be device memory,
+ * and stays in use by the graph until destroy() (the caller keeps it alive).
  */
 typedef struct t1m_batch t1m_batch;
 int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
@@ -150,6 +151,13 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
                      const char* lc, const t1m_options* opt, t1m_batch** out);
 int t1m_batch_launch(t1m_batch* batch, void* stream);
 int t1m_batch_destroy(t1m_batch* batch);
+/* What the batch's graph runs: its tiling mode and, per persistent launch,
+ * the tile family, instances, tiles and grid (or the per-instance launch
+ * count).  With opt->autotune, create() times the candidate tiling modes
+ * (which instances take the large tiles) on a scratch copy of C and keeps
+ * the fastest; the choice is kept in the tuning file for later batches of
+ * the same shapes. */
+const char* t1m_batch_describe(const t1m_batch* batch);
 
 /* ---- index maps (exported so they can be checked bit for bit) ---------- */
 

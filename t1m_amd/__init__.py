@@ -63,7 +63,7 @@ class T1MError(RuntimeError):
 
 # Exported symbols of include/t1m_hip.h.
 EXPORTS = ["t1m_options_init", "t1m_contract", "t1m_contract_noalloc", "t1m_contract_vjp",
-           "t1m_batch_create", "t1m_batch_launch", "t1m_batch_destroy",
+           "t1m_batch_create", "t1m_batch_launch", "t1m_batch_destroy", "t1m_batch_describe",
            "t1m_index_bundles", "t1m_index_maps", "t1m_pack_block", "t1m_unpack",
            "t1m_last_error", "t1m_version", "t1m_plan_cache_size",
            "t1m_plan_cache_clear", "t1m_autotune_cache_path", "t1m_plan_kernel"]
@@ -95,6 +95,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
                                    [P(Options), P(ctypes.c_void_p)])
     L.t1m_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.t1m_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.t1m_batch_describe.argtypes = [ctypes.c_void_p]
+    L.t1m_batch_describe.restype = ctypes.c_char_p
     L.t1m_index_bundles.argtypes = [ctypes.c_char_p] * 3 + [_i64p, _i64p]
     L.t1m_index_maps.argtypes = [P(TensorDesc), _i64p, ctypes.c_int32, ctypes.c_int64,
                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -296,7 +298,7 @@ class Batch:
 
     def __init__(self, a: Sequence, la: str, b: Sequence, lb: str, c: Sequence, lc: str,
                  alpha=1.0, precision="native", conj_a=False, conj_b=False,
-                 stream=None, synchronize: bool = True):
+                 stream=None, synchronize: bool = True, autotune: bool = False):
         n = len(a)
         if not (len(b) == n and len(c) == n):
             raise ValueError("a, b and c must have the same length")
@@ -311,7 +313,7 @@ class Batch:
         arr = lambda k: (TensorDesc * max(n, 1))(*[d[k].desc for d in self._keep])
         self._a, self._b, self._c = arr(0), arr(1), arr(2)
         o = _options(alpha, precision, bool(conj_a) ^ ca, bool(conj_b) ^ cb, stream,
-                     synchronize)
+                     synchronize, autotune)
         h = ctypes.c_void_p()
         _check(load().t1m_batch_create(n, self._a, la.encode(), self._b, lb.encode(),
                                        self._c, lc.encode(), ctypes.byref(o),
@@ -327,6 +329,10 @@ class Batch:
         if ops and all(cj for _, cj in ops):
             return [t for t, _ in ops], True
         return [t.conj().resolve_conj() if cj else t for t, cj in ops], False
+
+    def describe(self) -> str:
+        """Tiling mode and persistent launches of the captured graph."""
+        return load().t1m_batch_describe(self._h).decode() if getattr(self, "_h", None) else ""
 
     def launch(self, stream=None) -> None:
         if not getattr(self, "_h", None):

@@ -4,17 +4,24 @@
 // handles one pair of tensors per call (include/t1m/t1m.h:560-577).
 //
 // Creation builds every instance's index maps (one plan per distinct shape),
-// writes the instances' kernel arguments to one device array, and lists
-// every (instance, tile) pair, sorted by decreasing contracted extent (the
-// work of a tile is proportional to K: longest-processing-time-first order).
-// The graph holds a single persistent launch of a few workgroups per CU that
-// walk that list with a stride of the grid size.  When the instances cannot
-// share one kernel (mixed gather directions or offset widths, or the simple
-// kernel), the graph holds one launch per instance instead.
+// writes the instances' kernel arguments to device arrays, and lists every
+// (instance, tile) pair, sorted by decreasing contracted extent (the work of
+// a tile is proportional to K: longest-processing-time-first order).  The
+// graph holds a persistent launch of a few workgroups per CU that walk that
+// list with a stride of the grid size -- one launch per tile family when
+// some instances take larger tiles than the rest.  Which instances take
+// which family is a tiling mode; with `autotune` the candidate modes are
+// timed on a scratch copy of C and the fastest is kept (and remembered in
+// the tuning file).  When the instances cannot share one kernel (mixed
+// gather directions or offset widths, or the simple kernel), the graph
+// holds one launch per instance instead.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -22,26 +29,43 @@
 
 using namespace t1m_hip;
 
+// One persistent launch of a batch: a tile family, the kernel arguments of
+// the instances it covers and its LPT-ordered work list.
+struct BatchPhase {
+  int kernel = 0;
+  void* d_args = nullptr;
+  BatchItem* d_items = nullptr;
+  int nitems = 0;
+  int ninst = 0;
+  int grid = 0;
+};
+
 struct t1m_batch {
   std::vector<std::shared_ptr<Plan>> plans;
   std::vector<GemmProblem> problems;
-  void* d_args = nullptr;
-  BatchItem* d_items = nullptr;
-  int kernel = 0;
-  int nitems = 0;
-  int grid = 0;
+  std::vector<BatchPhase> phases;   // launched one after another
+  int mode = 0;                     // tiling mode of the phases
   bool persistent = false;
   hipStream_t capture = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int synchronize = 1;
-  ~t1m_batch() {
+  std::string description;
+  void release_device() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (capture) (void)hipStreamDestroy(capture);
-    if (d_args) (void)hipFree(d_args);
-    if (d_items) (void)hipFree(d_items);
+    for (BatchPhase& ph : phases) {
+      if (ph.d_args) (void)hipFree(ph.d_args);
+      if (ph.d_items) (void)hipFree(ph.d_items);
+    }
+    exec = nullptr;
+    graph = nullptr;
+    capture = nullptr;
+    phases.clear();
+    persistent = false;
   }
+  ~t1m_batch() { release_device(); }
 };
 
 namespace {
@@ -54,8 +78,9 @@ int cu_count_batch() {
   return cus > 0 ? cus : 256;
 }
 
-// Tile family for a whole batch: the small tiles, which keep the many small
-// instances of a sweep busy (one instance rarely fills the 144-wide tiles).
+// Tile family of a batch's instances: the small tiles, which keep the many
+// small instances of a sweep busy (one instance rarely fills the 144-wide
+// tiles).
 int batch_kernel(int dtype, int precision) {
   if (const char* f = env_override("T1M_FORCE_KERNEL")) {
     const int k = atoi(f);
@@ -79,46 +104,212 @@ int batch_kernel(int dtype, int precision) {
   }
 }
 
-int build_persistent(t1m_batch& bt, int precision) {
-  const size_t n = bt.problems.size();
-  if (n == 0) return T1M_SUCCESS;
-  const GemmProblem& g0 = bt.problems[0];
-  const int kern = batch_kernel(g0.dtype, precision);
-  if (kern == K_SIMPLE) return T1M_SUCCESS;
-  for (const GemmProblem& g : bt.problems)
-    if (g.a_mfast != g0.a_mfast || g.b_kfast != g0.b_kfast || g.idx32 != g0.idx32 ||
-        g.K >= (int64_t(1) << 31))
-      return T1M_SUCCESS;
+// Tiling modes (which instances run on which tile family):
+//   0  every instance on the batch family (batch_kernel);
+//   1  complex<float>: instances whose 144 x 144 tiling wastes < 10 % of its
+//      area and whose K >= 128 on the 144 x 144 tiles, the rest on 64 x 64;
+//   2  complex<float>: every instance at least 144 x 144 on the 144 x 144
+//      tiles, the rest on 64 x 64;
+//   3  complex<double>: every instance on the 64 x 64 tiles.
+// Measured on the 512-instance MPS sweep (bond dimensions 16..256, extents
+// multiples of 64): mode 0 84.7 TFLOP/s, mode 1 80.4, mode 2 67 -- ragged
+// 144-wide tiles and one 144 x 144 workgroup per CU lose there; batches of
+// large, evenly tiled instances gain from them.
+std::vector<int> tiling_modes(int dtype, int precision) {
+  if (env_override("T1M_FORCE_KERNEL")) return {0};
+  if (dtype == T1M_COMPLEX64 && precision == T1M_PRECISION_NATIVE) return {0, 1, 2};
+  if (dtype == T1M_COMPLEX128 && precision == T1M_PRECISION_NATIVE) return {0, 3};
+  return {0};
+}
+
+int kernel_for(const GemmProblem& g, int family, int mode) {
+  auto util = [](int64_t x) { return double(x) / double((x + 143) / 144 * 144); };
+  switch (mode) {
+    case 1:
+      return util(g.M) * util(g.N) >= 0.9 && g.K >= 128 ? K_C64_F32_144 : family;
+    case 2:
+      return g.M >= 144 && g.N >= 144 ? K_C64_F32_144 : family;
+    case 3:
+      return K_C128_F64_64;
+    default:
+      return family;
+  }
+}
+
+int build_phase(t1m_batch& bt, int kern, const std::vector<size_t>& which) {
+  if (which.empty()) return T1M_SUCCESS;
+  const GemmProblem& g0 = bt.problems[which[0]];
   const size_t asz = batch_args_size(g0.idx32 != 0);
-  std::vector<char> args(asz * n);
+  std::vector<char> args(asz * which.size());
   std::vector<BatchItem> items;
-  for (size_t i = 0; i < n; ++i) {
+  for (size_t w = 0; w < which.size(); ++w) {
     int tm = 0, tn = 0;
-    batch_make_args(bt.problems[i], kern, args.data() + asz * i, &tm, &tn);
-    const int32_t work = int32_t(bt.problems[i].K);
+    batch_make_args(bt.problems[which[w]], kern, args.data() + asz * w, &tm, &tn);
+    const int32_t work = int32_t(bt.problems[which[w]].K);
     for (int a = 0; a < tm; ++a)
-      for (int b = 0; b < tn; ++b) items.push_back({int32_t(i), a, b, work});
+      for (int b = 0; b < tn; ++b) items.push_back({int32_t(w), a, b, work});
   }
   std::stable_sort(items.begin(), items.end(),
                    [](const BatchItem& x, const BatchItem& y) { return x.work > y.work; });
   if (items.size() > size_t(INT32_MAX))
     return set_error(T1M_ERROR_UNSUPPORTED, "batch has too many tiles");
-  if (hipMalloc(&bt.d_args, args.size()) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void**>(&bt.d_items), items.size() * sizeof(BatchItem)) !=
+  bt.phases.emplace_back();
+  BatchPhase& ph = bt.phases.back();   // freed with the batch, also on failure
+  ph.kernel = kern;
+  ph.ninst = int(which.size());
+  if (hipMalloc(&ph.d_args, args.size()) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&ph.d_items), items.size() * sizeof(BatchItem)) !=
           hipSuccess) {
     (void)hipGetLastError();
     return set_error(T1M_ERROR_OUT_OF_MEMORY, "hipMalloc for the batch work list failed");
   }
-  HIP_TRY(hipMemcpy(bt.d_args, args.data(), args.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(bt.d_items, items.data(), items.size() * sizeof(BatchItem),
+  HIP_TRY(hipMemcpy(ph.d_args, args.data(), args.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ph.d_items, items.data(), items.size() * sizeof(BatchItem),
                     hipMemcpyHostToDevice));
   const int per_cu = batch_blocks_per_cu(kern, g0.idx32 != 0, g0.a_mfast != 0, g0.b_kfast != 0);
   const int64_t slots = int64_t(cu_count_batch()) * per_cu;
-  bt.grid = int(std::min<int64_t>(slots, int64_t(items.size())));
-  bt.nitems = int(items.size());
-  bt.kernel = kern;
+  ph.grid = int(std::min<int64_t>(slots, int64_t(items.size())));
+  ph.nitems = int(items.size());
+  return T1M_SUCCESS;
+}
+
+// The persistent phases for tiling mode `mode`, largest tiles first; none
+// when the instances cannot share one kernel.
+int build_persistent(t1m_batch& bt, int precision, int mode) {
+  const size_t n = bt.problems.size();
+  bt.mode = mode;
+  if (n == 0) return T1M_SUCCESS;
+  const GemmProblem& g0 = bt.problems[0];
+  const int family = batch_kernel(g0.dtype, precision);
+  if (family == K_SIMPLE) return T1M_SUCCESS;
+  for (const GemmProblem& g : bt.problems)
+    if (g.a_mfast != g0.a_mfast || g.b_kfast != g0.b_kfast || g.idx32 != g0.idx32 ||
+        g.K >= (int64_t(1) << 31))
+      return T1M_SUCCESS;
+  std::vector<int> kerns;
+  std::vector<std::vector<size_t>> groups;
+  for (size_t i = 0; i < n; ++i) {
+    const int k = kernel_for(bt.problems[i], family, mode);
+    size_t gi = 0;
+    while (gi < kerns.size() && kerns[gi] != k) ++gi;
+    if (gi == kerns.size()) {
+      kerns.push_back(k);
+      groups.emplace_back();
+    }
+    groups[gi].push_back(i);
+  }
+  // Larger tiles first (their items are longer).
+  std::vector<size_t> order(kerns.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    return kernel_info(kerns[x]).bm > kernel_info(kerns[y]).bm;
+  });
+  int rc;
+  for (size_t gi : order)
+    if ((rc = build_phase(bt, kerns[gi], groups[gi]))) return rc;
   bt.persistent = true;
   return T1M_SUCCESS;
+}
+
+// Captures the batch's launches (persistent phases, or one launch per
+// instance) into a graph and instantiates it.
+int capture_batch(t1m_batch& bt) {
+  HIP_TRY(hipStreamCreateWithFlags(&bt.capture, hipStreamNonBlocking));
+  HIP_TRY(hipStreamBeginCapture(bt.capture, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  if (bt.persistent) {
+    const GemmProblem& g0 = bt.problems[0];
+    for (size_t i = 0; i < bt.phases.size() && e == hipSuccess; ++i) {
+      const BatchPhase& ph = bt.phases[i];
+      e = launch_contract_batched(ph.kernel, g0.idx32 != 0, g0.a_mfast != 0, g0.b_kfast != 0,
+                                  ph.d_args, ph.d_items, ph.nitems, ph.grid, bt.capture);
+    }
+  } else {
+    for (size_t i = 0; i < bt.problems.size() && e == hipSuccess; ++i)
+      e = launch_contract(bt.problems[i], bt.plans[i]->kernel.load(), bt.capture);
+  }
+  if (e != hipSuccess) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(bt.capture, &g);
+    if (g) (void)hipGraphDestroy(g);
+    return set_error(T1M_ERROR_DEVICE, "batch capture failed: %s", hipGetErrorString(e));
+  }
+  HIP_TRY(hipStreamEndCapture(bt.capture, &bt.graph));
+  HIP_TRY(hipGraphInstantiate(&bt.exec, bt.graph, nullptr, nullptr, 0));
+  return T1M_SUCCESS;
+}
+
+// Key of a batch in the tuning file: device, dtype, precision and a hash of
+// every problem's shape and gather directions.
+std::string batch_key(const t1m_batch& bt, int precision) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](int64_t v) {
+    for (int i = 0; i < 8; ++i) h = (h ^ uint8_t(uint64_t(v) >> (8 * i))) * 1099511628211ull;
+  };
+  for (const GemmProblem& g : bt.problems) {
+    mix(g.M); mix(g.N); mix(g.K);
+    mix(g.a_mfast); mix(g.b_kfast); mix(g.idx32); mix(g.c_overlap);
+  }
+  char buf[160];
+  snprintf(buf, sizeof(buf), "batch|%s|dt%d|p%d|n%zu|h%016llx", device_tag().c_str(),
+           bt.problems.empty() ? 0 : bt.problems[0].dtype, precision, bt.problems.size(),
+           (unsigned long long)h);
+  return buf;
+}
+
+// Times tiling mode `mode` on a copy of the batch whose outputs are moved
+// into `scratch` (C's values there are arbitrary: only time is measured).
+int time_mode(const t1m_batch& bt, int precision, int mode,
+              const std::vector<std::ptrdiff_t>& shift, float* ms_out) {
+  t1m_batch trial;
+  trial.plans = bt.plans;
+  trial.problems = bt.problems;
+  for (size_t i = 0; i < trial.problems.size(); ++i)
+    trial.problems[i].C = static_cast<char*>(trial.problems[i].C) + shift[i];
+  int rc;
+  if ((rc = build_persistent(trial, precision, mode))) return rc;
+  if ((rc = capture_batch(trial))) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return set_error(T1M_ERROR_DEVICE, "hipEventCreate for batch tuning failed");
+  }
+  hipError_t e = hipGraphLaunch(trial.exec, trial.capture);   // warm-up
+  if (e == hipSuccess) e = hipGraphLaunch(trial.exec, trial.capture);
+  float best = 1e30f;
+  for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) {
+    e = hipEventRecord(e0, trial.capture);
+    if (e == hipSuccess) e = hipGraphLaunch(trial.exec, trial.capture);
+    if (e == hipSuccess) e = hipEventRecord(e1, trial.capture);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && ms < best) best = ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess)
+    return set_error(T1M_ERROR_DEVICE, "batch tuning launch failed: %s", hipGetErrorString(e));
+  *ms_out = best;
+  return T1M_SUCCESS;
+}
+
+void describe(t1m_batch& bt) {
+  char buf[128];
+  bt.description.clear();
+  if (!bt.persistent) {
+    snprintf(buf, sizeof(buf), "per-instance launches: %zu", bt.problems.size());
+    bt.description = buf;
+    return;
+  }
+  snprintf(buf, sizeof(buf), "mode %d", bt.mode);
+  bt.description = buf;
+  for (const BatchPhase& ph : bt.phases) {
+    snprintf(buf, sizeof(buf), "; %s: %d instances, %d tiles, grid %d",
+             kernel_info(ph.kernel).name, ph.ninst, ph.nitems, ph.grid);
+    bt.description += buf;
+  }
 }
 
 }  // namespace
@@ -140,7 +331,9 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
   bt->synchronize = o.synchronize;
   const hipStream_t stream = static_cast<hipStream_t>(o.stream);
   int rc;
-  std::vector<std::pair<uintptr_t, uintptr_t>> spans;   // C byte range of each instance
+  struct Span { uintptr_t lo, hi; };
+  std::vector<Span> spans;              // C byte range of each instance
+  std::vector<size_t> owner;            // instance (index into spans) of each problem
   for (int64_t i = 0; i < count; ++i) {
     Problem P;
     if ((rc = parse_problem(a + i, la, b + i, lb, c + i, lc, P))) return rc;
@@ -168,7 +361,7 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
     g.c_overlap = plan->c_overlap ? 1 : 0;
     const uintptr_t c0 = reinterpret_cast<uintptr_t>(c[i].data);
     const int64_t es = dtype_size(P.dtype);
-    spans.emplace_back(c0 + plan->c_lo * es, c0 + (plan->c_hi + 1) * es);
+    spans.push_back({c0 + plan->c_lo * es, c0 + (plan->c_hi + 1) * es});
     // An instance with batch labels contributes one problem per batch index.
     for (int64_t z = 0; z < plan->L; ++z) {
       GemmProblem gz = g;
@@ -180,42 +373,78 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
       if (plan->swap_ab) transpose_problem(gz);   // as the plan's kernel choice assumes
       bt->plans.push_back(plan);
       bt->problems.push_back(gz);
+      owner.push_back(spans.size() - 1);
     }
   }
   // Instances run side by side: when two write into one range of memory,
   // every instance adds atomically.
-  std::sort(spans.begin(), spans.end());
-  for (size_t i = 1; i < spans.size(); ++i)
-    if (spans[i].first < spans[i - 1].second) {
-      for (GemmProblem& g : bt->problems) g.c_overlap = 1;
-      break;
+  {
+    std::vector<Span> sorted = spans;
+    std::sort(sorted.begin(), sorted.end(),
+              [](const Span& x, const Span& y) { return x.lo < y.lo; });
+    for (size_t i = 1; i < sorted.size(); ++i)
+      if (sorted[i].lo < sorted[i - 1].hi) {
+        for (GemmProblem& g : bt->problems) g.c_overlap = 1;
+        break;
+      }
+  }
+
+  // Tiling mode: from the tuning file, else timed (autotune), else mode 0.
+  const std::vector<int> modes = tiling_modes(bt->problems.empty() ? 0 : bt->problems[0].dtype,
+                                              o.precision);
+  int mode = 0;
+  const char* forced = env_override("T1M_BATCH_MODE");   // tests / tuning
+  if (forced && std::find(modes.begin(), modes.end(), atoi(forced)) != modes.end()) {
+    mode = atoi(forced);
+  } else if (modes.size() > 1 && !bt->problems.empty()) {
+    const std::string key = batch_key(*bt, o.precision);
+    const int known = tuned_batch_mode(key);
+    if (known >= 0 && std::find(modes.begin(), modes.end(), known) != modes.end()) {
+      mode = known;
+    } else if (o.autotune) {
+      // Scratch outputs: each instance's C span, packed one after another.
+      std::vector<size_t> offs(spans.size());
+      size_t total = 0;
+      for (size_t i = 0; i < spans.size(); ++i) {
+        offs[i] = total;
+        total += (spans[i].hi - spans[i].lo + 255) / 256 * 256;
+      }
+      char* scratch = nullptr;
+      if (hipMalloc(&scratch, total) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(T1M_ERROR_OUT_OF_MEMORY,
+                         "hipMalloc of %zu bytes for batch tuning failed", total);
+      }
+      std::vector<std::ptrdiff_t> shift(bt->problems.size());
+      for (size_t p = 0; p < shift.size(); ++p) {
+        const Span& sp = spans[owner[p]];
+        shift[p] = std::ptrdiff_t(reinterpret_cast<uintptr_t>(scratch) + offs[owner[p]]) -
+                   std::ptrdiff_t(sp.lo);
+      }
+      // The scratch outputs sit packed in one allocation, the real ones
+      // wherever the caller put them: another mode replaces the default
+      // only when it is faster by 3 % or more.
+      float best = 1e30f;
+      for (int m : modes) {
+        float ms = 0;
+        rc = time_mode(*bt, o.precision, m, shift, &ms);
+        if (rc) break;
+        if (m == modes[0] || ms < 0.97f * best) { best = ms; mode = m; }
+      }
+      (void)hipFree(scratch);
+      if (rc) return rc;
+      store_batch_mode(key, mode);
     }
-  if ((rc = build_persistent(*bt, o.precision))) return rc;
+  }
+  if ((rc = build_persistent(*bt, o.precision, mode))) return rc;
   if (!bt->persistent && o.autotune) {
     for (size_t i = 0; i < bt->plans.size(); ++i)
       if (!bt->plans[i]->tuned.load() &&
           (rc = autotune_plan(*bt->plans[i], bt->problems[i], stream)))
         return rc;
   }
-  HIP_TRY(hipStreamCreateWithFlags(&bt->capture, hipStreamNonBlocking));
-  HIP_TRY(hipStreamBeginCapture(bt->capture, hipStreamCaptureModeThreadLocal));
-  hipError_t e = hipSuccess;
-  if (bt->persistent) {
-    const GemmProblem& g0 = bt->problems[0];
-    e = launch_contract_batched(bt->kernel, g0.idx32 != 0, g0.a_mfast != 0, g0.b_kfast != 0,
-                                bt->d_args, bt->d_items, bt->nitems, bt->grid, bt->capture);
-  } else {
-    for (size_t i = 0; i < bt->problems.size() && e == hipSuccess; ++i)
-      e = launch_contract(bt->problems[i], bt->plans[i]->kernel.load(), bt->capture);
-  }
-  if (e != hipSuccess) {
-    hipGraph_t g = nullptr;
-    (void)hipStreamEndCapture(bt->capture, &g);
-    if (g) (void)hipGraphDestroy(g);
-    return set_error(T1M_ERROR_DEVICE, "batch capture failed: %s", hipGetErrorString(e));
-  }
-  HIP_TRY(hipStreamEndCapture(bt->capture, &bt->graph));
-  HIP_TRY(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
+  if ((rc = capture_batch(*bt))) return rc;
+  describe(*bt);
   *out = bt.release();
   return T1M_SUCCESS;
 }
@@ -233,6 +462,10 @@ int t1m_batch_destroy(t1m_batch* batch) {
   clear_error();
   delete batch;
   return T1M_SUCCESS;
+}
+
+const char* t1m_batch_describe(const t1m_batch* batch) {
+  return batch ? batch->description.c_str() : "";
 }
 
 }  // extern "C"

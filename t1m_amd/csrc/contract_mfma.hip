@@ -449,8 +449,10 @@ k_contract_1m_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __re
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const BatchItem w = items[it];
     const KArgs<Idx> p = inst[w.inst];
-    contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF, sizeof(T) == 4>(p, w.tm, w.tn, 0,
-                                                                           int(p.K));
+    // C is prefetched before the main loop on the small FP32 tiles; the
+    // 144 x 144 tiles have no registers for it.
+    contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF, sizeof(T) == 4 && BM < 144>(
+        p, w.tm, w.tn, 0, int(p.K));
     __syncthreads();   // the next tile rewrites LDS
   }
 }
@@ -610,7 +612,8 @@ struct Batched1M {
 template <typename Idx, typename F>
 auto with_1m_config(int32_t kernel, F&& f) {
   switch (kernel) {
-    case K_C64_F32_144:   // batches run the small square tiles only
+    // The large tiles (a batch's large instances in tiling modes 1 and 2).
+    case K_C64_F32_144: return f(Batched1M<float, Idx, 144, 144, 16, 2, 4>());
     case K_C64_F32_THIN:
     // Stages of 8 k: half the LDS and 160 registers, 3 workgroups per CU
     // instead of 2 (+2 % on 512 MPS contractions; items are short in k).

@@ -101,6 +101,11 @@ size_t plan_device_bytes(const Plan& pl);
 hipError_t plan_build_maps(Plan& pl, const Problem& P, char* base, hipStream_t stream);
 // A tuning-file entry for the plan's key, if it fits the plan (host only).
 void apply_tuning_file(Plan& pl);
+// "<arch>/<CUs>" of the current device (tuning-file keys).
+std::string device_tag();
+// A batch's tiling mode from the tuning file (-1: none), and storing one.
+int tuned_batch_mode(const std::string& key);
+void store_batch_mode(const std::string& key, int mode);
 // The kernel arguments of a call of plan `pl` on device data A, B, C.
 GemmProblem gemm_for(const Plan& pl, const void* A, const void* B, void* C,
                      const t1m_options& opt);

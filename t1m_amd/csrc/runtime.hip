@@ -270,7 +270,8 @@ namespace {
 struct TuneCache {
   std::mutex mu;
   bool loaded = false;
-  std::unordered_map<std::string, int> best;   // key -> kernel id
+  std::unordered_map<std::string, int> best;          // key -> kernel id
+  std::unordered_map<std::string, int> batch_modes;   // batch key -> tiling mode
   std::string path;
 };
 
@@ -299,13 +300,18 @@ void load_tune_cache(TuneCache& T) {
   if (!f) return;
   char key[512], name[128];
   while (fscanf(f, "%511s %127s", key, name) == 2) {
+    if (strncmp(name, "batch_mode_", 11) == 0) {
+      const int m = atoi(name + 11);
+      if (m >= 0 && m < 16) T.batch_modes[key] = m;
+      continue;
+    }
     const int k = kernel_by_name(name);
     if (k >= 0) T.best[key] = k;
   }
   fclose(f);
 }
 
-void append_tune_cache(TuneCache& T, const std::string& key, int kernel) {
+void append_tune_line(TuneCache& T, const std::string& key, const char* value) {
   // Create the file's directory without a shell: mkdir each prefix (a bare
   // file name lives in the working directory).
   const size_t slash = T.path.find_last_of('/');
@@ -314,9 +320,43 @@ void append_tune_cache(TuneCache& T, const std::string& key, int kernel) {
     if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
   FILE* f = fopen(T.path.c_str(), "a");
   if (!f) return;
-  fprintf(f, "%s %s\n", key.c_str(), kernel_info(kernel).name);
+  fprintf(f, "%s %s\n", key.c_str(), value);
   fclose(f);
 }
+
+void append_tune_cache(TuneCache& T, const std::string& key, int kernel) {
+  append_tune_line(T, key, kernel_info(kernel).name);
+}
+
+}  // namespace
+
+std::string device_tag() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  std::string arch = "gpu";
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) arch = prop.gcnArchName;
+  for (char& ch : arch) if (ch == ':' || ch == ' ') ch = '_';
+  return arch + "/" + std::to_string(cu_count());
+}
+
+int tuned_batch_mode(const std::string& key) {
+  TuneCache& T = tune_cache();
+  std::lock_guard<std::mutex> lock(T.mu);
+  load_tune_cache(T);
+  auto it = T.batch_modes.find(key);
+  return it == T.batch_modes.end() ? -1 : it->second;
+}
+
+void store_batch_mode(const std::string& key, int mode) {
+  TuneCache& T = tune_cache();
+  std::lock_guard<std::mutex> lock(T.mu);
+  load_tune_cache(T);
+  T.batch_modes[key] = mode;
+  append_tune_line(T, key, ("batch_mode_" + std::to_string(mode)).c_str());
+}
+
+namespace {
 
 // A kernel id read from the tuning file is used only if it computes the
 // plan's dtype at its precision and the plan's K fits the MFMA kernels' int
@@ -329,15 +369,9 @@ bool tuned_kernel_ok(const Plan& pl, int kernel) {
 }
 
 std::string tune_key(const Plan& pl) {
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  hipDeviceProp_t prop;
-  std::string arch = "gpu";
-  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) arch = prop.gcnArchName;
-  for (char& ch : arch) if (ch == ':' || ch == ' ') ch = '_';
   char buf[256];
-  snprintf(buf, sizeof(buf), "%s/%d|dt%d|p%d|M%lld|N%lld|K%lld|a%d|b%d|i%d", arch.c_str(),
-           cu_count(), pl.dtype, pl.precision, (long long)pl.M, (long long)pl.N,
+  snprintf(buf, sizeof(buf), "%s|dt%d|p%d|M%lld|N%lld|K%lld|a%d|b%d|i%d", device_tag().c_str(),
+           pl.dtype, pl.precision, (long long)pl.M, (long long)pl.N,
            (long long)pl.K, pl.a_mfast, pl.b_kfast, pl.idx32);
   if (pl.L > 1) {
     const size_t n = strlen(buf);
@@ -820,6 +854,7 @@ void plan_cache_clear() {
     std::lock_guard<std::mutex> lock(T.mu);
     T.loaded = false;
     T.best.clear();
+    T.batch_modes.clear();
   }
   drop_parked_plans();
 }

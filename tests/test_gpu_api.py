@@ -224,3 +224,92 @@ def test_batch_keeps_temporary_inputs_alive():
     with pytest.raises(RuntimeError):
         batch.launch()
     del junk
+
+
+def _batch_case(rng, shapes):
+    As, Bs, Cs, C0s = [], [], [], []
+    for m, k, n in shapes:
+        A = rand(rng, (m, k), np.complex64)
+        B = rand(rng, (k, n), np.complex64)
+        C0 = rand(rng, (m, n), np.complex64)
+        As.append(to_dev(A)); Bs.append(to_dev(B)); Cs.append(to_dev(C0)); C0s.append(C0)
+    return As, Bs, Cs, C0s
+
+
+def _check_batch(As, Bs, Cs, C0s, tol=TOL32):
+    for a, b, c, c0 in zip(As, Bs, Cs, C0s):
+        want = (a.to(torch.complex128) @ b.to(torch.complex128)).cpu().numpy()
+        assert rel_err(to_host(c) - c0, want) < tol
+
+
+# Large instances tiled evenly by 144 x 144 (576 = 4 x 144, K >= 128) beside
+# small ones: every tiling mode puts different instances on the large tiles.
+MIXED_SHAPES = [(576, 256, 576), (288, 160, 432), (576, 128, 288),
+                (64, 32, 64), (96, 48, 80), (128, 16, 64), (200, 64, 150)]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_batch_tiling_modes_compute_the_same(mode):
+    """Each tiling mode of a complex<float> batch (all 64 x 64; evenly tiled
+    large instances on 144 x 144; every instance >= 144 x 144 on 144 x 144)
+    gives the products, on instances of both kinds, against complex128."""
+    rng = np.random.default_rng(107 + mode)
+    As, Bs, Cs, C0s = _batch_case(rng, MIXED_SHAPES)
+    old = os.environ.get("T1M_BATCH_MODE")
+    os.environ["T1M_BATCH_MODE"] = str(mode)
+    try:
+        bt = t1m_amd.Batch(As, "ik", Bs, "kj", Cs, "ij")
+    finally:
+        if old is None:
+            os.environ.pop("T1M_BATCH_MODE", None)
+        else:
+            os.environ["T1M_BATCH_MODE"] = old
+    desc = bt.describe()
+    print(" ", desc)
+    assert desc.startswith(f"mode {mode}")
+    if mode > 0:
+        assert "c64_f32mfma_144x144" in desc
+    bt.launch()
+    torch.cuda.synchronize()
+    _check_batch(As, Bs, Cs, C0s)
+    bt.close()
+
+
+def test_batch_autotune_times_modes_and_remembers(tmp_path):
+    """autotune=True times the tiling modes on scratch outputs (C is touched
+    once, by the kept graph), writes the choice to the tuning file, and a
+    later batch of the same shapes reads it back."""
+    path = tmp_path / "tuning.txt"
+    old = os.environ.get("T1M_TUNE_CACHE")
+    os.environ["T1M_TUNE_CACHE"] = str(path)
+    try:
+        t1m_amd.plan_cache_clear()
+        rng = np.random.default_rng(108)
+        As, Bs, Cs, C0s = _batch_case(rng, MIXED_SHAPES)
+        bt = t1m_amd.Batch(As, "ik", Bs, "kj", Cs, "ij", autotune=True)
+        torch.cuda.synchronize()
+        for c, c0 in zip(Cs, C0s):                     # tuning wrote to scratch only
+            assert np.array_equal(to_host(c), c0)
+        desc = bt.describe()
+        print(" ", desc)
+        lines = [ln.split() for ln in path.read_text().splitlines() if "batch|" in ln]
+        assert len(lines) == 1 and lines[0][1].startswith("batch_mode_"), lines
+        mode = int(lines[0][1][len("batch_mode_"):])
+        assert desc.startswith(f"mode {mode}")
+        bt.launch()
+        torch.cuda.synchronize()
+        _check_batch(As, Bs, Cs, C0s)
+        bt.close()
+        As2, Bs2, Cs2, C0s2 = _batch_case(rng, MIXED_SHAPES)
+        bt2 = t1m_amd.Batch(As2, "ik", Bs2, "kj", Cs2, "ij")
+        assert bt2.describe().startswith(f"mode {mode}")
+        bt2.launch()
+        torch.cuda.synchronize()
+        _check_batch(As2, Bs2, Cs2, C0s2)
+        bt2.close()
+    finally:
+        if old is None:
+            os.environ.pop("T1M_TUNE_CACHE", None)
+        else:
+            os.environ["T1M_TUNE_CACHE"] = old
+        t1m_amd.plan_cache_clear()

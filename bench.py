@@ -130,6 +130,24 @@ def cold_call_ms(t1m, torch, a, b, c, cfg, precision):
     return (time.perf_counter() - t0) * 1e3
 
 
+def host_staged(t1m, torch, a, b, c, cfg, fl, calls=3):
+    """The same contraction on host (CPU) tensors: each call copies A, B and
+    C to the device, contracts, copies C back and waits (PCIe-inclusive; a
+    convenience path, not the metric).  Best of `calls` after one warm call."""
+    ah, bh, ch = a.cpu(), b.cpu(), c.cpu()
+    t1m.contract(ah, cfg["la"], bh, cfg["lb"], ch, cfg["lc"])
+    best = 1e30
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        t1m.contract(ah, cfg["la"], bh, cfg["lb"], ch, cfg["lc"])
+        best = min(best, time.perf_counter() - t0)
+    es = a.element_size()
+    moved = (a.numel() + b.numel() + 2 * c.numel()) * es
+    return {"ms_per_call": round(best * 1e3, 3), "value": round(fl / best / 1e12, 3),
+            "unit": "TFLOP/s", "pcie_bytes_per_call": moved,
+            "pcie_GB_per_s": round(moved / best / 1e9, 2)}
+
+
 def time_path(t1m, torch, a, b, c, cfg, precision, steps, warmup):
     stream = torch.cuda.current_stream()
     cold = cold_call_ms(t1m, torch, a, b, c, cfg, precision)
@@ -711,6 +729,8 @@ def config_result(name, args, torch, t1m, headline):
                 "dtype": dtype, "workload": cfg["workload"], "flop_per_step": fl,
                 "roofline": roof}
     line["cold_call_ms"] = round(cold, 3)
+    if headline:
+        line["host_staged"] = host_staged(t1m, torch, a, b, c, cfg, fl)
     if cfg["dtype"] == "complex64":
         c2 = torch.zeros_like(c)
         wall_b, kb, _, kname_b, _ = time_path(t1m, torch, a, b, c2, cfg, "bf16x3", args.steps,

@@ -221,7 +221,7 @@ const char* t1m_plan_kernel(const t1m_tensor* a, const char* la,
                             const t1m_options* opt);
 /* Number of plans in the plan cache (index maps kept on the device). */
 int64_t t1m_plan_cache_size(void);
-/* Drop every cached plan and free its device memory. */
+/* Drop every cached plan and free its device memory (and the pooled host-staging buffers). */
 void t1m_plan_cache_clear(void);
 
 #ifdef __cplusplus

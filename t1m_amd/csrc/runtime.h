@@ -66,9 +66,17 @@ struct Plan {
   ~Plan();
 };
 
-// A host tensor copied through device memory for one call.
+// A host tensor copied through device memory for one call.  The device
+// buffer comes from a per-device pool of staging buffers and goes back to
+// it when the call ends; a call that did not reach its final stream
+// synchronisation (an error path) synchronises before giving it back, so no
+// queued copy or kernel still uses a pooled buffer.
 struct Staged {
   void* buf = nullptr;
+  size_t cap = 0;                // pooled buffer's size (>= bytes)
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  bool synced = false;           // the stream was synchronised after the last use
   void* device = nullptr;        // element-0 address inside buf
   const char* host_base = nullptr;
   size_t bytes = 0;
@@ -78,6 +86,8 @@ struct Staged {
   int copy_back(void* host, hipStream_t stream);
   ~Staged();
 };
+// Frees every pooled staging buffer (t1m_plan_cache_clear).
+void staging_pool_clear();
 
 int dtype_size(int dtype);
 int check_tensor(const t1m_tensor* t, const char* labels, const char* what,

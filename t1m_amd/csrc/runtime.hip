@@ -857,6 +857,7 @@ void plan_cache_clear() {
     T.batch_modes.clear();
   }
   drop_parked_plans();
+  staging_pool_clear();
 }
 
 // A call captured into a graph bakes the plan's map pointers (and batch
@@ -906,9 +907,95 @@ bool is_device_pointer(const void* ptr) {
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
+namespace {
+
+// Staging buffers kept between calls, per device: a call with host tensors
+// takes the smallest free buffer that fits (else allocates one) and returns
+// it afterwards, so repeated host-pointer calls pay no hipMalloc / hipFree
+// (hipFree also waits for the whole device).  At most kPoolBytes stay
+// cached per device; larger returns are freed.
+constexpr size_t kPoolBytes = size_t(4) << 30;
+struct StagingPool {
+  std::mutex mu;
+  std::vector<std::vector<std::pair<void*, size_t>>> free;   // [device] -> (buffer, size)
+  std::vector<size_t> held;                                  // [device] cached bytes
+};
+StagingPool& staging_pool() {
+  static StagingPool* p = new StagingPool();
+  return *p;
+}
+
+void* pool_take(int dev, size_t bytes, size_t& cap) {
+  StagingPool& P = staging_pool();
+  {
+    std::lock_guard<std::mutex> lock(P.mu);
+    if (size_t(dev) < P.free.size()) {
+      auto& v = P.free[size_t(dev)];
+      size_t best = v.size();
+      for (size_t i = 0; i < v.size(); ++i)
+        if (v[i].second >= bytes && (best == v.size() || v[i].second < v[best].second)) best = i;
+      if (best < v.size()) {
+        void* b = v[best].first;
+        cap = v[best].second;
+        P.held[size_t(dev)] -= cap;
+        v.erase(v.begin() + std::ptrdiff_t(best));
+        return b;
+      }
+    }
+  }
+  void* b = nullptr;
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    staging_pool_clear();                  // cached buffers may be what is missing
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  cap = bytes;
+  return b;
+}
+
+void pool_give(int dev, void* b, size_t cap) {
+  StagingPool& P = staging_pool();
+  {
+    std::lock_guard<std::mutex> lock(P.mu);
+    if (size_t(dev) >= P.free.size()) {
+      P.free.resize(size_t(dev) + 1);
+      P.held.resize(size_t(dev) + 1, 0);
+    }
+    if (P.held[size_t(dev)] + cap <= kPoolBytes) {
+      P.free[size_t(dev)].emplace_back(b, cap);
+      P.held[size_t(dev)] += cap;
+      return;
+    }
+  }
+  (void)hipFree(b);
+}
+
+}  // namespace
+
+void staging_pool_clear() {
+  std::vector<std::vector<std::pair<void*, size_t>>> drop;
+  {
+    StagingPool& P = staging_pool();
+    std::lock_guard<std::mutex> lock(P.mu);
+    drop.swap(P.free);
+    P.held.assign(P.held.size(), 0);
+  }
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (size_t d = 0; d < drop.size(); ++d) {
+    if (drop[d].empty()) continue;
+    (void)hipSetDevice(int(d));
+    for (auto& [b, n] : drop[d]) (void)hipFree(b);
+  }
+  (void)hipSetDevice(cur);
+}
+
 int Staged::stage(const void* host, const std::vector<int64_t>& dims,
                   const std::vector<int64_t>& strides, int esize, bool copy_in,
-                  hipStream_t stream) {
+                  hipStream_t stream_) {
   int64_t lo = 0, hi = 0;
   bool empty = false;
   for (size_t i = 0; i < dims.size(); ++i) {
@@ -919,11 +1006,11 @@ int Staged::stage(const void* host, const std::vector<int64_t>& dims,
   host_base = static_cast<const char*>(host) + lo * esize;
   bytes = empty ? 0 : size_t(hi - lo + 1) * esize;
   if (bytes == 0) { device = const_cast<void*>(host); return T1M_SUCCESS; }
-  if (hipMalloc(&buf, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    buf = nullptr;
+  HIP_TRY(hipGetDevice(&dev));
+  stream = stream_;
+  buf = pool_take(dev, bytes, cap);
+  if (!buf)
     return set_error(T1M_ERROR_OUT_OF_MEMORY, "hipMalloc of %zu bytes for staging failed", bytes);
-  }
   device = static_cast<char*>(buf) - lo * esize;
   if (copy_in) HIP_TRY(hipMemcpyAsync(buf, host_base, bytes, hipMemcpyHostToDevice, stream));
   return T1M_SUCCESS;
@@ -937,7 +1024,9 @@ int Staged::copy_back(void* host, hipStream_t stream) {
 }
 
 Staged::~Staged() {
-  if (buf) (void)hipFree(buf);
+  if (!buf) return;
+  if (!synced) (void)hipStreamSynchronize(stream);   // error paths: nothing may still use it
+  pool_give(dev, buf, cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1095,7 +1184,10 @@ int contract_impl(const t1m_tensor* a, const char* la, const t1m_tensor* b,
     return set_error(T1M_ERROR_DEVICE, "kernel %s launch failed: %s",
                      kernel_info(plan->kernel.load()).name, hipGetErrorString(e));
   if (hostC && (rc = sc.copy_back(c->data, stream))) return rc;
-  if (opt.synchronize || hostA || hostB || hostC) HIP_TRY(hipStreamSynchronize(stream));
+  if (opt.synchronize || hostA || hostB || hostC) {
+    HIP_TRY(hipStreamSynchronize(stream));
+    sa.synced = sb.synced = sc.synced = true;
+  }
   return T1M_SUCCESS;
 }
 

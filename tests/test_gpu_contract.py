@@ -482,6 +482,33 @@ def test_host_pointers_are_staged():
     assert rel_err(C - C0, want - C0) < TOL["fp32"]
 
 
+def test_host_staging_buffers_are_reused():
+    """Host-pointer calls take their device buffers from a per-device pool:
+    a larger call after smaller ones, a smaller one reusing a larger buffer
+    (stale bytes beyond its extent must not matter), a pool emptied by a
+    plan cache clear, and host operands mixed with device ones."""
+    rng = np.random.default_rng(12)
+    for step, (a_, b_, c_, d_) in enumerate([(12, 7, 9, 11), (40, 33, 21, 17), (5, 3, 4, 2),
+                                             (40, 33, 21, 17), (9, 8, 7, 6)]):
+        if step == 3:
+            t1m_amd.plan_cache_clear()
+        A = rand(rng, (a_, b_, c_), np.complex64)
+        B = rand(rng, (b_, d_), np.complex64)
+        C = rand(rng, (a_, c_, d_), np.complex64)
+        C0 = C.copy()
+        want = oracle_einsum(A, "abc", B, "bd", C0, "acd")
+        if step == 4:   # device A, host B and C
+            a = to_dev(A)
+            c = torch.from_numpy(C)
+            t1m_amd.contract(a, "abc", torch.from_numpy(B), "bd", c, "acd")
+            got = c.numpy()
+        else:
+            c = torch.from_numpy(C)
+            t1m_amd.contract(torch.from_numpy(A), "abc", torch.from_numpy(B), "bd", c, "acd")
+            got = c.numpy()
+        assert rel_err(got - C0, want - C0) < TOL["fp32"], step
+
+
 def test_autotune_picks_a_candidate_and_persists():
     rng = np.random.default_rng(21)
     la, lb, lc = "abcd", "dbef", "acef"

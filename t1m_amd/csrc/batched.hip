@@ -35,9 +35,11 @@ struct BatchPhase {
   int kernel = 0;
   void* d_args = nullptr;
   BatchItem* d_items = nullptr;
-  int nitems = 0;
+  int nitems = 0;    // work-list entries, padding included
+  int ntiles = 0;    // tiles (entries that are not padding)
   int ninst = 0;
   int grid = 0;
+  int xcds = 1;      // L2 domains the work list is ordered for
 };
 
 struct t1m_batch {
@@ -136,21 +138,96 @@ int kernel_for(const GemmProblem& g, int family, int mode) {
   }
 }
 
+// L2 domains (XCDs) of the device: gfx950 and gfx942 parts dispatch the
+// workgroups of a launch round-robin over their XCDs (workgroup b on XCD
+// b mod n), each with its own L2.  T1M_BATCH_XCDS overrides (1: no XCD
+// ordering).
+int xcd_count() {
+  if (const char* e = env_override("T1M_BATCH_XCDS")) {
+    const int n = atoi(e);
+    return n >= 1 && n <= 64 ? n : 1;
+  }
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 1;
+  }
+  const int cus = cu_count_batch();
+  if (strncmp(prop.gcnArchName, "gfx950", 6) == 0 && cus % 8 == 0) return 8;
+  if (strncmp(prop.gcnArchName, "gfx942", 6) == 0) return cus % 8 == 0 ? 8 : cus % 6 == 0 ? 6 : 1;
+  return 1;
+}
+
+// Work list ordered for n L2 domains.  Each instance goes whole to one
+// domain (instances by decreasing work, each to the least loaded domain),
+// so its A and B panels are fetched into one L2 instead of every domain's;
+// each domain's items run longest first (LPT) and the domains' lists are
+// interleaved, entry p going to domain p mod n, which is where workgroup
+// p mod grid runs when the grid is a multiple of n.  Shorter lists are
+// padded with entries of instance -1, which the kernels skip.
+std::vector<BatchItem> xcd_order(const std::vector<BatchItem>& items, size_t ninst,
+                                 const std::vector<int64_t>& inst_work, int n) {
+  std::vector<size_t> order(ninst);
+  for (size_t i = 0; i < ninst; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t x, size_t y) { return inst_work[x] > inst_work[y]; });
+  std::vector<int> dom(ninst, 0);
+  std::vector<int64_t> load(static_cast<size_t>(n), 0);
+  std::vector<size_t> rank(ninst, 0);
+  for (size_t r = 0; r < ninst; ++r) {
+    const size_t i = order[r];
+    int best = 0;
+    for (int d = 1; d < n; ++d)
+      if (load[size_t(d)] < load[size_t(best)]) best = d;
+    dom[i] = best;
+    load[size_t(best)] += inst_work[i];
+    rank[i] = r;
+  }
+  std::vector<std::vector<BatchItem>> lists(static_cast<size_t>(n));
+  std::vector<BatchItem> sorted = items;   // instance-major, then by rank
+  std::stable_sort(sorted.begin(), sorted.end(), [&](const BatchItem& x, const BatchItem& y) {
+    return rank[size_t(x.inst)] < rank[size_t(y.inst)];
+  });
+  for (const BatchItem& it : sorted) lists[size_t(dom[size_t(it.inst)])].push_back(it);
+  size_t longest = 0;
+  for (auto& l : lists) {
+    std::stable_sort(l.begin(), l.end(),
+                     [](const BatchItem& x, const BatchItem& y) { return x.work > y.work; });
+    longest = std::max(longest, l.size());
+  }
+  std::vector<BatchItem> out(longest * size_t(n), BatchItem{-1, 0, 0, 0});
+  for (int d = 0; d < n; ++d)
+    for (size_t k = 0; k < lists[size_t(d)].size(); ++k)
+      out[k * size_t(n) + size_t(d)] = lists[size_t(d)][k];
+  return out;
+}
+
 int build_phase(t1m_batch& bt, int kern, const std::vector<size_t>& which) {
   if (which.empty()) return T1M_SUCCESS;
   const GemmProblem& g0 = bt.problems[which[0]];
   const size_t asz = batch_args_size(g0.idx32 != 0);
   std::vector<char> args(asz * which.size());
   std::vector<BatchItem> items;
+  std::vector<int64_t> inst_work(which.size(), 0);
   for (size_t w = 0; w < which.size(); ++w) {
     int tm = 0, tn = 0;
     batch_make_args(bt.problems[which[w]], kern, args.data() + asz * w, &tm, &tn);
     const int32_t work = int32_t(bt.problems[which[w]].K);
+    inst_work[w] = int64_t(tm) * tn * work;
     for (int a = 0; a < tm; ++a)
       for (int b = 0; b < tn; ++b) items.push_back({int32_t(w), a, b, work});
   }
-  std::stable_sort(items.begin(), items.end(),
-                   [](const BatchItem& x, const BatchItem& y) { return x.work > y.work; });
+  const size_t ntiles = items.size();
+  const int per_cu = batch_blocks_per_cu(kern, g0.idx32 != 0, g0.a_mfast != 0, g0.b_kfast != 0);
+  const int64_t slots = int64_t(cu_count_batch()) * per_cu;
+  const int nx = xcd_count();
+  if (nx > 1 && slots % nx == 0) {
+    items = xcd_order(items, which.size(), inst_work, nx);
+  } else {
+    std::stable_sort(items.begin(), items.end(),
+                     [](const BatchItem& x, const BatchItem& y) { return x.work > y.work; });
+  }
   if (items.size() > size_t(INT32_MAX))
     return set_error(T1M_ERROR_UNSUPPORTED, "batch has too many tiles");
   bt.phases.emplace_back();
@@ -166,10 +243,10 @@ int build_phase(t1m_batch& bt, int kern, const std::vector<size_t>& which) {
   HIP_TRY(hipMemcpy(ph.d_args, args.data(), args.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ph.d_items, items.data(), items.size() * sizeof(BatchItem),
                     hipMemcpyHostToDevice));
-  const int per_cu = batch_blocks_per_cu(kern, g0.idx32 != 0, g0.a_mfast != 0, g0.b_kfast != 0);
-  const int64_t slots = int64_t(cu_count_batch()) * per_cu;
   ph.grid = int(std::min<int64_t>(slots, int64_t(items.size())));
   ph.nitems = int(items.size());
+  ph.ntiles = int(ntiles);
+  ph.xcds = (nx > 1 && slots % nx == 0) ? nx : 1;
   return T1M_SUCCESS;
 }
 
@@ -306,8 +383,8 @@ void describe(t1m_batch& bt) {
   snprintf(buf, sizeof(buf), "mode %d", bt.mode);
   bt.description = buf;
   for (const BatchPhase& ph : bt.phases) {
-    snprintf(buf, sizeof(buf), "; %s: %d instances, %d tiles, grid %d",
-             kernel_info(ph.kernel).name, ph.ninst, ph.nitems, ph.grid);
+    snprintf(buf, sizeof(buf), "; %s: %d instances, %d tiles, grid %d, %d L2 domains",
+             kernel_info(ph.kernel).name, ph.ninst, ph.ntiles, ph.grid, ph.xcds);
     bt.description += buf;
   }
 }

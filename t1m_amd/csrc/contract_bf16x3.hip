@@ -404,6 +404,7 @@ k_contract_bf16x3_batched(const KArgs<Idx>* __restrict__ inst,
                           const BatchItem* __restrict__ items, int nitems) {
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const BatchItem w = items[it];
+    if (w.inst < 0) continue;   // padding of an XCD-ordered work list
     const KArgs<Idx> p = inst[w.inst];
     contract_tile_bf16x3<Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn, 0, int(p.K));
     __syncthreads();

@@ -448,6 +448,7 @@ k_contract_1m_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __re
                       int nitems) {
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const BatchItem w = items[it];
+    if (w.inst < 0) continue;   // padding of an XCD-ordered work list
     const KArgs<Idx> p = inst[w.inst];
     // C is prefetched before the main loop on the small FP32 tiles; the
     // 144 x 144 tiles have no registers for it.

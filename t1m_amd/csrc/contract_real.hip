@@ -239,6 +239,7 @@ k_contract_real_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __
                         int nitems) {
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const BatchItem w = items[it];
+    if (w.inst < 0) continue;   // padding of an XCD-ordered work list
     const KArgs<Idx> p = inst[w.inst];
     contract_tile_real<T, Idx, BM, BN, BK, WGM, WGN, AF, BF>(p, w.tm, w.tn, 0, int(p.K));
     __syncthreads();

@@ -275,6 +275,44 @@ def test_batch_tiling_modes_compute_the_same(mode):
     bt.close()
 
 
+@pytest.mark.parametrize("xcds", ["1", "8", "3"])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_xcd_ordered_work_lists(xcds, mode):
+    """Work lists ordered for several L2 domains (whole instances per
+    domain, interleaved, padded with skipped entries) give the products of
+    the plain LPT list.  3 domains divide a grid of 3 workgroups per CU
+    (the 64 x 64 phase) but not the 144 x 144 phase's one per CU, which then
+    keeps the plain list."""
+    rng = np.random.default_rng(109)
+    As, Bs, Cs, C0s = _batch_case(rng, MIXED_SHAPES)
+    saved = {k: os.environ.get(k) for k in ("T1M_BATCH_MODE", "T1M_BATCH_XCDS")}
+    os.environ["T1M_BATCH_MODE"] = str(mode)
+    os.environ["T1M_BATCH_XCDS"] = xcds
+    try:
+        bt = t1m_amd.Batch(As, "ik", Bs, "kj", Cs, "ij")
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    desc = bt.describe()
+    print(" ", desc)
+    if xcds == "3":
+        assert "3 L2 domains" in desc or "1 L2 domains" in desc
+    else:
+        assert desc.count(f"{xcds} L2 domains") == desc.count("grid")
+    bt.launch()
+    torch.cuda.synchronize()
+    _check_batch(As, Bs, Cs, C0s)
+    bt.launch()                 # a second launch adds the products again
+    torch.cuda.synchronize()
+    for a, b, c, c0 in zip(As, Bs, Cs, C0s):
+        want = 2 * (a.to(torch.complex128) @ b.to(torch.complex128)).cpu().numpy()
+        assert rel_err(to_host(c) - c0, want) < 2e-6
+    bt.close()
+
+
 def test_batch_autotune_times_modes_and_remembers(tmp_path):
     """autotune=True times the tiling modes on scratch outputs (C is touched
     once, by the kept graph), writes the choice to the tuning file, and a

@@ -110,7 +110,12 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   __shared__ C2 sB[2][BK][LDB];
 
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wid = tid >> 6;
+  // With a shared column block the wave index is made scalar, so the
+  // per-row-block test below is a scalar branch rather than an exec-mask one
+  // (+0.6..0.8 % at 144 x 144; on the 64 x 64 tiles, which have no such
+  // test, the scalar index measured -1 %).
+  const int lane = tid & 63;
+  const int wid = XT ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
   const int wm = wid % WGM, wn = wid / WGM;
   // Shared column block: n-wave wn takes row blocks [xlo, xhi), MB / WGN
   // each and one more for the first MB % WGN waves.  Waves wn and wn + 2

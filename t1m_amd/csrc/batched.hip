@@ -40,6 +40,7 @@ struct BatchPhase {
   int ninst = 0;
   int grid = 0;
   int xcds = 1;      // L2 domains the work list is ordered for
+  bool pairs = false;  // every instance allows two-element gathers (complex<float>)
 };
 
 struct t1m_batch {
@@ -219,7 +220,10 @@ int build_phase(t1m_batch& bt, int kern, const std::vector<size_t>& which) {
       for (int b = 0; b < tn; ++b) items.push_back({int32_t(w), a, b, work});
   }
   const size_t ntiles = items.size();
-  const int per_cu = batch_blocks_per_cu(kern, g0.idx32 != 0, g0.a_mfast != 0, g0.b_kfast != 0);
+  bool pairs = true;
+  for (size_t w : which) pairs = pairs && bt.problems[w].a_pairs && bt.problems[w].b_pairs;
+  const int per_cu =
+      batch_blocks_per_cu(kern, g0.idx32 != 0, g0.a_mfast != 0, g0.b_kfast != 0, pairs);
   const int64_t slots = int64_t(cu_count_batch()) * per_cu;
   const int nx = xcd_count();
   if (nx > 1 && slots % nx == 0) {
@@ -247,6 +251,7 @@ int build_phase(t1m_batch& bt, int kern, const std::vector<size_t>& which) {
   ph.nitems = int(items.size());
   ph.ntiles = int(ntiles);
   ph.xcds = (nx > 1 && slots % nx == 0) ? nx : 1;
+  ph.pairs = pairs;
   return T1M_SUCCESS;
 }
 
@@ -299,7 +304,8 @@ int capture_batch(t1m_batch& bt) {
     for (size_t i = 0; i < bt.phases.size() && e == hipSuccess; ++i) {
       const BatchPhase& ph = bt.phases[i];
       e = launch_contract_batched(ph.kernel, g0.idx32 != 0, g0.a_mfast != 0, g0.b_kfast != 0,
-                                  ph.d_args, ph.d_items, ph.nitems, ph.grid, bt.capture);
+                                  ph.pairs, ph.d_args, ph.d_items, ph.nitems, ph.grid,
+                                  bt.capture);
     }
   } else {
     for (size_t i = 0; i < bt.problems.size() && e == hipSuccess; ++i)
@@ -432,6 +438,7 @@ int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,
     g.alpha_re = o.alpha_re; g.alpha_im = cplx ? o.alpha_im : 0.0;
     g.conj_a = cplx && o.conj_a; g.conj_b = cplx && o.conj_b;
     g.a_mfast = plan->a_mfast; g.b_kfast = plan->b_kfast;
+    g.a_pairs = plan->a_pairs; g.b_pairs = plan->b_pairs;
     g.idx32 = plan->idx32; g.dtype = P.dtype;
     g.split_k = 1;   // whole tiles per workgroup (no split K)
     g.cus = 0;

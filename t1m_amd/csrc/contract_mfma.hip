@@ -55,8 +55,10 @@ namespace {
 
 // One BM x BN tile (tm, tn) of one contraction, whole K range.  PREC: load
 // C's offsets and values before the main loop (see the epilogue set-up).
+// PAIRS (complex<float>): both operands gathered two neighbours per lane
+// (PairGather).
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
-          bool AF, bool BF, bool PREC>
+          bool AF, bool BF, bool PREC, bool PAIRS = false>
 __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, int tn, int kbeg, int kend) {
   using C2 = typename Vec<T>::c2;
   using Acc = typename Vec<T>::acc;
@@ -135,8 +137,11 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
 
   // ---- gathers: A tile (BM x BK), B tile (BN x BK) -----------------------
   const unsigned utid = unsigned(tid);
-  using GA = TileGather<C2, Idx, BM, BK, NT, AF>;
-  using GB = TileGather<C2, Idx, BN, BK, NT, BF>;
+  static_assert(!PAIRS || sizeof(T) == 4, "pairs of complex<float>");
+  using GA = std::conditional_t<PAIRS, PairGather<Idx, BM, BK, NT, AF>,
+                                TileGather<C2, Idx, BM, BK, NT, AF>>;
+  using GB = std::conditional_t<PAIRS, PairGather<Idx, BN, BK, NT, BF>,
+                                TileGather<C2, Idx, BN, BK, NT, BF>>;
   GA ga;
   GB gb;
   ga.init(utid, i0, M, p.rsA);
@@ -146,31 +151,36 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
 
   // Data registers of a stage: one set, or two when the loads run two
   // stages ahead (DEEP).
-  using VA = C2[GA::E];
-  using VB = C2[GB::E];
+  using VA = typename GA::V[GA::E];
+  using VB = typename GB::V[GB::E];
   auto swrite = [&](int buf, int p0, const VA& va, const VB& vb) {
 #pragma unroll
     for (int u = 0; u < GA::E; ++u) {
-      const unsigned e = GA::elem(utid, u);
-      const int k = GA::kc(e);
       if (GA::valid(utid, u)) {
-        const C2 v = ga.finished(utid, p0, K, sa, u, va[u]);
-        if constexpr (AEXP) {
-          AExp w;
-          w.e.x = v.x; w.e.y = -v.y;
-          w.o.x = v.y; w.o.y = v.x;
-          sA[buf][k][GA::xc(e) ^ swa(k)] = w;
-        } else {
-          sA[buf][k][GA::xc(e)] = v;
+#pragma unroll
+        for (int j = 0; j < GA::PER; ++j) {
+          const int k = GA::kcj(utid, u, j), x = GA::xcj(utid, u, j);
+          const C2 v = ga.finished(utid, p0, K, sa, u, j, va[u]);
+          if constexpr (AEXP) {
+            AExp w;
+            w.e.x = v.x; w.e.y = -v.y;
+            w.o.x = v.y; w.o.y = v.x;
+            sA[buf][k][x ^ swa(k)] = w;
+          } else {
+            sA[buf][k][x] = v;
+          }
         }
       }
     }
 #pragma unroll
     for (int u = 0; u < GB::E; ++u) {
-      const unsigned e = GB::elem(utid, u);
-      const int k = GB::kc(e);
-      if (GB::valid(utid, u))
-        sB[buf][k][GB::xc(e) ^ swb(k)] = gb.finished(utid, p0, K, sb, u, vb[u]);
+      if (GB::valid(utid, u)) {
+#pragma unroll
+        for (int j = 0; j < GB::PER; ++j) {
+          const int k = GB::kcj(utid, u, j), x = GB::xcj(utid, u, j);
+          sB[buf][k][x ^ swb(k)] = gb.finished(utid, p0, K, sb, u, j, vb[u]);
+        }
+      }
     }
   };
 
@@ -435,19 +445,19 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
 }
 
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
-          bool AF, bool BF>
+          bool AF, bool BF, bool PAIRS>
 __global__ void __launch_bounds__(64 * WGM * WGN)
 k_contract_1m(KArgs<Idx> p) {
   int tm, tn, kbeg, kend;
   tile_and_krange(p, tm, tn, kbeg, kend);
-  contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF, false>(p, tm, tn, kbeg, kend);
+  contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF, false, PAIRS>(p, tm, tn, kbeg, kend);
 }
 
 // Persistent batched form: workgroup g takes items g, g + G, g + 2G, ... of
 // a work list of (instance, tile) sorted by decreasing work, so the grid
 // (a few workgroups per CU) drains it in LPT order in one launch.
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
-          bool AF, bool BF>
+          bool AF, bool BF, bool PAIRS>
 __global__ void __launch_bounds__(64 * WGM * WGN)
 k_contract_1m_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __restrict__ items,
                       int nitems) {
@@ -457,7 +467,7 @@ k_contract_1m_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __re
     const KArgs<Idx> p = inst[w.inst];
     // C is prefetched before the main loop on the small FP32 tiles; the
     // 144 x 144 tiles have no registers for it.
-    contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF, sizeof(T) == 4 && BM < 144>(
+    contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF, sizeof(T) == 4 && BM < 144, PAIRS>(
         p, w.tm, w.tn, 0, int(p.K));
     __syncthreads();   // the next tile rewrites LDS
   }
@@ -517,31 +527,46 @@ hipError_t launch_simple(const GemmProblem& g, hipStream_t s) {
   });
 }
 
-template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN>
+// The kernel for the gather directions (and, with ALLOW_PAIRS, two-element
+// gathers when both operands allow them).
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool ALLOW_PAIRS>
+using Kernel1M = void (*)(KArgs<Idx>);
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN, bool ALLOW_PAIRS>
+Kernel1M<T, Idx, BM, BN, BK, WGM, WGN, ALLOW_PAIRS> pick_1m(bool af, bool bf, bool pairs) {
+  if constexpr (ALLOW_PAIRS) {
+    if (pairs) {
+      if (af && bf) return k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, true, true>;
+      if (af) return k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, false, true>;
+      if (bf) return k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, true, true>;
+      return k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, false, true>;
+    }
+  }
+  if (af && bf) return k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, true, false>;
+  if (af) return k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, false, false>;
+  if (bf) return k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, true, false>;
+  return k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, false, false>;
+}
+
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
+          bool ALLOW_PAIRS = false>
 hipError_t launch_1m(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, BM, BN, BK);
   const int64_t per = int64_t(a.tiles_m) * a.tiles_n * a.ksplit;
   const dim3 block(64 * WGM * WGN);
-  const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
+  const auto kern = pick_1m<T, Idx, BM, BN, BK, WGM, WGN, ALLOW_PAIRS>(
+      g.a_mfast != 0, g.b_kfast == 0, g.a_pairs && g.b_pairs);
   return launch_batches(a, per, 64 * WGM * WGN, [&](const KArgs<Idx>& q, unsigned grid) {
-    if (af && bf)
-      hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, q);
-    else if (af)
-      hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, true, false>), grid, block, 0, s, q);
-    else if (bf)
-      hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, true>), grid, block, 0, s, q);
-    else
-      hipLaunchKernelGGL((k_contract_1m<T, Idx, BM, BN, BK, WGM, WGN, false, false>), grid, block, 0, s, q);
+    hipLaunchKernelGGL(kern, grid, block, 0, s, q);
   });
 }
 
 template <typename Idx>
 hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
   switch (kernel) {
-    case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, 16, 2, 4>(g, s);
+    case K_C64_F32_144: return launch_1m<float, Idx, 144, 144, 16, 2, 4, true>(g, s);
     // 8-k stages: 3 workgroups per CU (+5 % and +13 % on two 48^4 / 24^4
     // layouts, -1 % on a third).
-    case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 8, 2, 2>(g, s);
+    case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 8, 2, 2, true>(g, s);
     // 4 x 2 waves (16 x 32 complex per wave, 2 waves per SIMD): +23 % over
     // 2 x 2 at 1024 x 1024 x 256, +3 % at 2304^3.
     case K_C128_F64_64: return launch_1m<double, Idx, 64, 64, 16, 4, 2>(g, s);
@@ -589,27 +614,36 @@ hipError_t launch_bf16x3_batched(int32_t kernel, bool idx32, bool af, bool bf,
 
 namespace {
 
-template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN>
+template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
+          bool ALLOW_PAIRS = false>
 struct Batched1M {
   using K = void (*)(const KArgs<Idx>*, const BatchItem*, int);
-  static K pick(bool af, bool bf) {
-    if (af && bf) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, true, true>;
-    if (af) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, true, false>;
-    if (bf) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, false, true>;
-    return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, false, false>;
+  static K pick(bool af, bool bf, bool pairs) {
+    if constexpr (ALLOW_PAIRS) {
+      if (pairs) {
+        if (af && bf) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, true, true, true>;
+        if (af) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, true, false, true>;
+        if (bf) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, false, true, true>;
+        return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, false, false, true>;
+      }
+    }
+    if (af && bf) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, true, true, false>;
+    if (af) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, true, false, false>;
+    if (bf) return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, false, true, false>;
+    return k_contract_1m_batched<T, Idx, BM, BN, BK, WGM, WGN, false, false, false>;
   }
-  static int per_cu(bool af, bool bf) {
+  static int per_cu(bool af, bool bf, bool pairs) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pick(af, bf), 64 * WGM * WGN, 0) !=
-        hipSuccess) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pick(af, bf, pairs), 64 * WGM * WGN,
+                                                     0) != hipSuccess) {
       (void)hipGetLastError();
       return 1;
     }
     return n > 0 ? n : 1;
   }
-  static hipError_t launch(bool af, bool bf, const void* inst, const BatchItem* items,
-                           int nitems, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(pick(af, bf), dim3(grid), dim3(64 * WGM * WGN), 0, s,
+  static hipError_t launch(bool af, bool bf, bool pairs, const void* inst,
+                           const BatchItem* items, int nitems, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(pick(af, bf, pairs), dim3(grid), dim3(64 * WGM * WGN), 0, s,
                        static_cast<const KArgs<Idx>*>(inst), items, nitems);
     return hipGetLastError();
   }
@@ -619,11 +653,11 @@ template <typename Idx, typename F>
 auto with_1m_config(int32_t kernel, F&& f) {
   switch (kernel) {
     // The large tiles (a batch's large instances in tiling modes 1 and 2).
-    case K_C64_F32_144: return f(Batched1M<float, Idx, 144, 144, 16, 2, 4>());
+    case K_C64_F32_144: return f(Batched1M<float, Idx, 144, 144, 16, 2, 4, true>());
     case K_C64_F32_THIN:
     // Stages of 8 k: half the LDS and 160 registers, 3 workgroups per CU
     // instead of 2 (+2 % on 512 MPS contractions; items are short in k).
-    case K_C64_F32_64: return f(Batched1M<float, Idx, 64, 64, 8, 2, 2>());
+    case K_C64_F32_64: return f(Batched1M<float, Idx, 64, 64, 8, 2, 2, true>());
     case K_C128_F64_64:
     case K_C128_F64_64W: return f(Batched1M<double, Idx, 64, 64, 16, 2, 2>());
     default: return f(Batched1M<double, Idx, 32, 32, 16, 2, 2>());
@@ -652,16 +686,17 @@ void batch_make_args(const GemmProblem& g0, int32_t kernel, void* out, int* tile
   }
 }
 
-int batch_blocks_per_cu(int32_t kernel, bool idx32, bool af, bool bk) {
+int batch_blocks_per_cu(int32_t kernel, bool idx32, bool af, bool bk, bool pairs) {
   const bool bf = !bk;
   if (kernel == K_C64_BF16X3_144 || kernel == K_C64_BF16X3_64)
     return bf16x3_batched_blocks_per_cu(kernel, idx32, af, bf);
   if (is_real_kernel(kernel)) return real_batched_blocks_per_cu(kernel, idx32, af, bf);
-  if (idx32) return with_1m_config<int32_t>(kernel, [&](auto c) { return c.per_cu(af, bf); });
-  return with_1m_config<int64_t>(kernel, [&](auto c) { return c.per_cu(af, bf); });
+  if (idx32)
+    return with_1m_config<int32_t>(kernel, [&](auto c) { return c.per_cu(af, bf, pairs); });
+  return with_1m_config<int64_t>(kernel, [&](auto c) { return c.per_cu(af, bf, pairs); });
 }
 
-hipError_t launch_contract_batched(int32_t kernel, bool idx32, bool af, bool bk,
+hipError_t launch_contract_batched(int32_t kernel, bool idx32, bool af, bool bk, bool pairs,
                                    const void* inst, const BatchItem* items, int nitems,
                                    int grid, hipStream_t s) {
   const bool bf = !bk;
@@ -672,10 +707,10 @@ hipError_t launch_contract_batched(int32_t kernel, bool idx32, bool af, bool bk,
     return launch_real_batched(kernel, idx32, af, bf, inst, items, nitems, grid, s);
   if (idx32)
     return with_1m_config<int32_t>(kernel, [&](auto c) {
-      return c.launch(af, bf, inst, items, nitems, grid, s);
+      return c.launch(af, bf, pairs, inst, items, nitems, grid, s);
     });
   return with_1m_config<int64_t>(kernel, [&](auto c) {
-    return c.launch(af, bf, inst, items, nitems, grid, s);
+    return c.launch(af, bf, pairs, inst, items, nitems, grid, s);
   });
 }
 

@@ -63,6 +63,9 @@ struct GemmProblem {
   int32_t split_k;   // 0: split K when the tiles cannot fill the CUs; 1: never
   int32_t cus;       // compute units of the device (split-K sizing)
   int32_t c_overlap = 0;   // distinct outputs may share an address: add atomically
+  // complex<float>: the operand's gather-direction map has map[i + 1] =
+  // map[i] + 1 for every even i, so lanes may load neighbour pairs.
+  int32_t a_pairs = 0, b_pairs = 0;
   // Batch modes (labels in A, B and C): L independent problems, problem z
   // reading A, B, C at byte offsets boffA[z], boffB[z], boffC[z] (device).
   int64_t L = 1;
@@ -116,10 +119,11 @@ size_t batch_args_size(bool idx32);
 void batch_make_args(const GemmProblem& g, int32_t kernel, void* out,
                      int* tiles_m, int* tiles_n);
 // Resident workgroups per CU of the batched form of `kernel`.
-int batch_blocks_per_cu(int32_t kernel, bool idx32, bool a_mfast, bool b_kfast);
+// pairs: every instance allows two-element gathers of both operands.
+int batch_blocks_per_cu(int32_t kernel, bool idx32, bool a_mfast, bool b_kfast, bool pairs);
 // Launches the persistent batched kernel over `nitems` work items.
 hipError_t launch_contract_batched(int32_t kernel, bool idx32, bool a_mfast,
-                                   bool b_kfast, const void* inst,
+                                   bool b_kfast, bool pairs, const void* inst,
                                    const BatchItem* items, int nitems, int grid,
                                    hipStream_t stream);
 

@@ -239,6 +239,15 @@ struct TileGather {
     v.y = ok ? sign * v.y : T(0);
     return v;
   }
+  // Per-element view shared with PairGather: slot u holds PER = 1 element.
+  static constexpr int PER = 1;
+  using V = T2;
+  __device__ static int xcj(unsigned tid, int u, int) { return xc(elem(tid, u)); }
+  __device__ static int kcj(unsigned tid, int u, int) { return kc(elem(tid, u)); }
+  template <typename T>
+  __device__ T2 finished(unsigned tid, int p0, int K, T sign, int u, int, T2 v) const {
+    return finished(tid, p0, K, sign, u, v);
+  }
   // Zero out-of-range entries (real data).
   __device__ void finish_real(unsigned tid, int p0, int K) {
 #pragma unroll
@@ -260,6 +269,82 @@ struct TileGather {
   __device__ void finish(unsigned tid, int p0, int K, T sign) {
 #pragma unroll
     for (int u = 0; u < E; ++u) finish_one(tid, p0, K, sign, u);
+  }
+};
+
+// complex<float> tile gather with two neighbours along the operand's unit-
+// stride direction per lane: slot u holds the pair (x, x+1) at one k (XF)
+// or (k, k+1) at one x, read with one 16-byte load.  Valid when the gather
+// direction's map has map[i + 1] = map[i] + 1 for every even i (leading mode
+// of unit stride and even extent; the extent of the direction is then even,
+// so a clamped pair stays inside the tensor) -- half the load instructions
+// and address arithmetic of TileGather for the same bytes.
+typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+
+template <typename Idx, int BX, int BK, int NT, bool XF>
+struct PairGather {
+  static_assert(BX % 2 == 0 && BK % 2 == 0, "pairs");
+  static constexpr int PX = XF ? BX / 2 : BX;   // pair grid
+  static constexpr int PK = XF ? BK : BK / 2;
+  static constexpr int TOTAL = PX * PK;
+  static constexpr int E = (TOTAL + NT - 1) / NT;
+  static constexpr bool RAGGED = (TOTAL % NT) != 0;
+  static constexpr int PER = 2;
+  static_assert(2 * E <= 32, "validity bits");
+  using V = f4a8;   // (re, im) of the pair's first element, then of the second
+  Idx xoff[E];
+  Idx koff[E];
+  V val[E];
+  uint32_t xok;     // bit 2u + j: element j of slot u has its x inside
+
+  __device__ static bool valid(unsigned tid, int u) {
+    return !RAGGED || u + 1 < E || tid + unsigned(NT) * u < unsigned(TOTAL);
+  }
+  __device__ static unsigned elem(unsigned tid, int u) {
+    return valid(tid, u) ? tid + unsigned(NT) * u : 0u;
+  }
+  __device__ static int xcj(unsigned tid, int u, int j) {
+    const unsigned e = elem(tid, u);
+    return XF ? 2 * int(e % unsigned(PX)) + j : int(e / unsigned(PK));
+  }
+  __device__ static int kcj(unsigned tid, int u, int j) {
+    const unsigned e = elem(tid, u);
+    return XF ? int(e / unsigned(PX)) : 2 * int(e % unsigned(PK)) + j;
+  }
+  __device__ void init(unsigned tid, int64_t x0, int64_t X, const Idx* xmap) {
+    xok = 0;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int64_t xg = x0 + xcj(tid, u, 0);
+      // an out-of-range pair reads the last pair (XF: X is even) or row
+      xoff[u] = xmap[xg < X ? xg : (XF ? X - 2 : X - 1)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int64_t xj = x0 + xcj(tid, u, j);
+        xok |= ((xj < X && valid(tid, u)) ? 1u : 0u) << (2 * u + j);
+      }
+    }
+  }
+  __device__ void load_k(unsigned tid, int p0, int K, const Idx* kmap) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int kg = p0 + kcj(tid, u, 0);
+      koff[u] = kmap[kg < K ? kg : (XF ? K - 1 : K - 2)];
+    }
+  }
+  __device__ void load_data(const float2* __restrict__ base) { load_data_to(base, val); }
+  __device__ void load_data_to(const float2* __restrict__ base, V (&out)[E]) const {
+#pragma unroll
+    for (int u = 0; u < E; ++u) out[u] = *reinterpret_cast<const V*>(base + (xoff[u] + koff[u]));
+  }
+  // Element j of slot u of a stage starting at p0: zero when out of range,
+  // conjugated if asked (sign = -1).
+  template <typename T>
+  __device__ float2 finished(unsigned tid, int p0, int K, T sign, int u, int j, V v) const {
+    const int kg = p0 + kcj(tid, u, j);
+    const bool ok = ((xok >> (2 * u + j)) & 1u) && kg < K;
+    const float re = j == 0 ? v.x : v.z, im = j == 0 ? v.y : v.w;
+    return float2{ok ? re : 0.f, ok ? float(sign) * im : 0.f};
   }
 };
 

@@ -56,6 +56,7 @@ struct Plan {
   bool swap_ab = false;                          // run as C^T = B^T A^T (thin m, see get_plan)
   int64_t M = 0, N = 0, K = 0, L = 1;
   int a_mfast = 1, b_kfast = 1, idx32 = 1;
+  int a_pairs = 0, b_pairs = 0;                  // two-element gathers allowed (GemmProblem)
   void* maps = nullptr;                          // owned device memory (null: borrowed)
   // Batch problems (L > 1): byte offsets into A, B, C, device [3][L] and host.
   int64_t* boff = nullptr;

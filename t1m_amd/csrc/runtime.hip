@@ -509,8 +509,9 @@ std::string plan_key(const Problem& P, int precision, int device) {
   // Test / tuning overrides change the plan, so they are part of its key.
   const char* fk = env_override("T1M_FORCE_KERNEL");
   const char* f64 = env_override("T1M_FORCE_IDX64");
-  snprintf(buf, sizeof(buf), "%d|%d|%d|%s|%s|", device, P.dtype, precision,
-           fk ? fk : "", f64 ? f64 : "");
+  const char* fp = env_override("T1M_PAIRS");
+  snprintf(buf, sizeof(buf), "%d|%d|%d|%.8s|%.8s|%.8s|", device, P.dtype, precision,
+           fk ? fk : "", f64 ? f64 : "", fp ? fp : "");
   k += buf;
   k += P.la; k += ','; k += P.lb; k += ','; k += P.lc; k += '|';
   auto add = [&](const std::vector<int64_t>& v) {
@@ -628,6 +629,7 @@ void transpose_problem(GemmProblem& g) {
   g.a_mfast = g.b_kfast ? 0 : 1;         // B gathered along n <=> B^T along its rows
   g.b_kfast = amf ? 0 : 1;               // A gathered along k <=> A^T along k
   std::swap(g.boffA, g.boffB);
+  std::swap(g.a_pairs, g.b_pairs);       // a property of each tensor's gather map
 }
 
 // True unless the layout provably gives every index tuple its own element:
@@ -706,6 +708,21 @@ void plan_layout(const Problem& P0, int precision, Plan& pl, Problem& P) {
   if (const char* f = env_override("T1M_FORCE_IDX64"))
     if (atoi(f) != 0) pl.idx32 = 0;
   pl.modes_m = P.m; pl.modes_n = P.n; pl.modes_k = P.k;
+  // Two-element gathers (complex<float>): the leading mode (extent > 1) of
+  // the operand's gather direction has unit stride and even extent, so
+  // map[i + 1] = map[i] + 1 for even i and the direction's extent is even.
+  auto unit_pairs = [](const std::vector<Mode>& modes, int which) {
+    for (const Mode& md : modes) {
+      if (md.dim <= 1) continue;
+      const int64_t st = which == 0 ? md.sa : md.sb;
+      return st == 1 && md.dim % 2 == 0;
+    }
+    return false;
+  };
+  const char* pe = env_override("T1M_PAIRS");   // tests: 0 keeps one-element gathers
+  const bool c64 = P.dtype == T1M_COMPLEX64 && !(pe && atoi(pe) == 0);
+  pl.a_pairs = c64 && unit_pairs(pl.a_mfast ? P.m : P.k, 0) ? 1 : 0;
+  pl.b_pairs = c64 && unit_pairs(pl.b_kfast ? P.k : P.n, 1) ? 1 : 0;
 }
 
 // Device bytes of a plan's maps: rsA, rsC (M entries), csA, rsB (K), csB,
@@ -1056,7 +1073,7 @@ void call_signature(const t1m_tensor* const* t, const char* const* l, int precis
   sig.push_back(int64_t(g_plan_generation.load()));
   sig.push_back(dev);
   sig.push_back(precision);
-  for (const char* env : {"T1M_FORCE_KERNEL", "T1M_FORCE_IDX64"}) {
+  for (const char* env : {"T1M_FORCE_KERNEL", "T1M_FORCE_IDX64", "T1M_PAIRS"}) {
     const char* v = env_override(env);
     int64_t h = v ? 1469598103934665603LL : 0;
     if (v)
@@ -1089,6 +1106,7 @@ GemmProblem gemm_for(const Plan& pl, const void* A, const void* B, void* C,
   g.conj_a = cplx && opt.conj_a; g.conj_b = cplx && opt.conj_b;
   if (!cplx) g.alpha_im = 0.0;
   g.a_mfast = pl.a_mfast; g.b_kfast = pl.b_kfast;
+  g.a_pairs = pl.a_pairs; g.b_pairs = pl.b_pairs;
   g.idx32 = pl.idx32;
   g.dtype = pl.dtype;
   g.split_k = opt.deterministic ? 1 : 0;

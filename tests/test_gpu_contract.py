@@ -1138,3 +1138,52 @@ def test_offsets_beyond_int32_take_64bit_maps():
     finally:
         del buf
         torch.cuda.empty_cache()
+
+
+PAIR_CASES = [
+    # (labels, extents): both operands pairable (unit stride, even leading
+    # extent) in all four gather-direction combinations, ragged tiles and K
+    (("ik", "kj", "ij"), {"i": 150, "j": 161, "k": 48}),
+    (("ki", "kj", "ij"), {"i": 146, "j": 70, "k": 34}),
+    (("ik", "jk", "ij"), {"i": 98, "j": 66, "k": 50}),
+    (("ki", "jk", "ij"), {"i": 40, "j": 290, "k": 18}),
+    (("abcd", "dbef", "acef"), {"a": 12, "b": 6, "c": 11, "d": 8, "e": 10, "f": 7}),
+    (("asb", "btc", "astc"), {"a": 38, "s": 4, "b": 30, "t": 4, "c": 22}),
+    # a batch label and an odd leading extent (pairs on one operand only)
+    (("zik", "zkj", "zij"), {"z": 3, "i": 70, "k": 16, "j": 33}),
+    (("ik", "kj", "ij"), {"i": 71, "j": 64, "k": 40}),
+]
+
+
+@pytest.mark.parametrize("case", range(len(PAIR_CASES)))
+def test_two_element_gathers(case):
+    """complex<float> gathers of two neighbours per lane (16-byte loads),
+    taken when an operand's gather direction leads with a unit-stride mode of
+    even extent, against the oracle and against the one-element gathers
+    (T1M_PAIRS=0), on both FP32 tile families, the thin tiles and BF16x3."""
+    (la, lb, lc), ext = PAIR_CASES[case]
+    rng = np.random.default_rng(300 + case)
+    A = rand(rng, [ext[x] for x in la], np.complex64)
+    B = rand(rng, [ext[x] for x in lb], np.complex64)
+    C0 = rand(rng, [ext[x] for x in lc], np.complex64)
+    want = oracle_einsum(A, la, B, lb, C0, lc)
+    for kern, prec, tolkey in (("c64_f32mfma_144x144", "native", "fp32"),
+                               ("c64_f32mfma_64x64", "native", "fp32"),
+                               ("c64_bf16x3_144x144", "bf16x3", "bf16x3")):
+        got = {}
+        for pairs in ("1", "0"):
+            old = os.environ.get("T1M_PAIRS")
+            os.environ["T1M_PAIRS"] = pairs
+            try:
+                with forced(kern):
+                    got[pairs] = run(A, la, B, lb, C0, lc, precision=prec)
+            finally:
+                if old is None:
+                    os.environ.pop("T1M_PAIRS", None)
+                else:
+                    os.environ["T1M_PAIRS"] = old
+            err = rel_err(got[pairs] - C0, want - C0)
+            print(f"  {la},{lb}->{lc} {kern} pairs={pairs} rel={err:.2e}")
+            assert err < TOL[tolkey], (kern, pairs, err)
+        # the same products in the same order: identical sums
+        assert np.array_equal(got["1"], got["0"]), kern

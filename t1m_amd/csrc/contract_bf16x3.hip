@@ -419,9 +419,11 @@ k_contract_bf16x3_batched(const KArgs<Idx>* __restrict__ inst,
 // are and, for every B element, the two words that pair with them,
 // (re, -im) for the real part of C and (im, re) for the imaginary part
 // (sum_p a_re b_re - a_im b_im and a_re b_im + a_im b_re).  Conjugating A
-// or B only flips sign bits of those two words (two xor masks), and the
-// words are formed once, when B is staged, so the MFMA operands come
-// straight from LDS.  A lane's outputs are whole complex elements, C[i][j]
+// or B only flips sign bits of those two words (two xor masks).  LDS holds
+// B's first form only: the second is its half-swapped copy with another sign
+// mask, formed in registers when a fragment is read (two VALU operations a
+// word, beside the MFMA stream), which takes a third of the LDS writes and a
+// quarter of the fragment reads off the LDS.  A lane's outputs are whole complex elements, C[i][j]
 // for i = l16 of a 16-row unit and j = 4 g + t of a 16-column unit.
 //
 // The tile is 9 x 9 units of 16 complex rows x 16 columns.  Each wave owns
@@ -444,7 +446,7 @@ constexpr WaveRect kRect144[8] = {
 constexpr int kP144 = 144;             // tile rows / columns
 constexpr int kPW = 16;                // words per LDS row (16 complex k)
 constexpr int kPPlane = kP144 * kPW;   // words per plane
-constexpr int kPPlanes = 6;            // A hi, A lo, B re-hi, B re-lo, B im-hi, B im-lo
+constexpr int kPPlanes = 4;            // A hi, A lo, B re-hi, B re-lo
 
 // Word offset of word w of LDS row x (chunk swizzle).
 __device__ inline int pword(int x, int w) {
@@ -481,6 +483,14 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
   // of A and of B flip the signs of the paired terms).
   const uint32_t mre = (p.conj_a != 0) == (p.conj_b != 0) ? 0x80000000u : 0u;
   const uint32_t mim = (p.conj_b ? 0x00008000u : 0u) | (p.conj_a ? 0x80000000u : 0u);
+  // im-form from the stored re-form r = w ^ mre: swap(r) = swap(w) ^ swap(mre).
+  const uint32_t mim_r = mim ^ (mre ? 0x00008000u : 0u);
+  auto im_form = [&](const u32x4& r) {
+    u32x4 x;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) x[t] = __builtin_amdgcn_alignbit(r[t], r[t], 16) ^ mim_r;
+    return x;
+  };
 
   // Stage slot u of a gather (values v) into LDS: zero out-of-range entries
   // (edge tiles and the last stage only), split into hi / lo, store (A raw,
@@ -502,18 +512,14 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
     if (G::valid(utid, u)) {
       const int w = pword(G::xc(e), 4 * G::kq(e));
       if (is_b) {
-        u32x4 rh, rl, ih, il;
+        u32x4 rh, rl;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           rh[t] = hi[t] ^ mre;
           rl[t] = lo[t] ^ mre;
-          ih[t] = __builtin_amdgcn_alignbit(hi[t], hi[t], 16) ^ mim;
-          il[t] = __builtin_amdgcn_alignbit(lo[t], lo[t], 16) ^ mim;
         }
         *reinterpret_cast<u32x4*>(plane(buf, 2) + w) = rh;
         *reinterpret_cast<u32x4*>(plane(buf, 3) + w) = rl;
-        *reinterpret_cast<u32x4*>(plane(buf, 4) + w) = ih;
-        *reinterpret_cast<u32x4*>(plane(buf, 5) + w) = il;
       } else {
         *reinterpret_cast<u32x4*>(plane(buf, 0) + w) = hi;
         *reinterpret_cast<u32x4*>(plane(buf, 1) + w) = lo;
@@ -531,8 +537,6 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
       if (is_b) {
         plane(buf, 2)[w] = hi ^ mre;
         plane(buf, 3)[w] = lo ^ mre;
-        plane(buf, 4)[w] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ mim;
-        plane(buf, 5)[w] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ mim;
       } else {
         plane(buf, 0)[w] = hi;
         plane(buf, 1)[w] = lo;
@@ -587,8 +591,8 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
     acc[r][c][1] = mfma_bf16(ih, ah, acc[r][c][1]);
   };
   // One k-chunk: the fragments of one dimension stay in registers -- B's
-  // (four planes, 16 registers a column) when they take no more registers
-  // than A's (two planes, 8 a row).
+  // (two forms of two planes, 16 registers a column) when they take no more
+  // registers than A's (two planes, 8 a row).
   constexpr bool HOLD_B = 2 * NC <= NR;
   constexpr int NS = HOLD_B ? NR : NC;     // streamed
   constexpr int NH = HOLD_B ? NC : NR;     // held
@@ -600,8 +604,8 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
         const int row = (c0 + x) * 16 + l16;
         h0[x] = frag(plane(buf, 2), row);
         h1[x] = frag(plane(buf, 3), row);
-        h2[x] = frag(plane(buf, 4), row);
-        h3[x] = frag(plane(buf, 5), row);
+        h2[x] = im_form(h0[x]);
+        h3[x] = im_form(h1[x]);
       } else {
         const int row = (r0 + x) * 16 + l16;
         h0[x] = frag(plane(buf, 0), row);
@@ -618,7 +622,7 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
       } else {
         const int row = (c0 + y) * 16 + l16;
         const u32x4 rh = frag(plane(buf, 2), row), rl = frag(plane(buf, 3), row);
-        const u32x4 ih = frag(plane(buf, 4), row), il = frag(plane(buf, 5), row);
+        const u32x4 ih = im_form(rh), il = im_form(rl);
 #pragma unroll
         for (int x = 0; x < NH; ++x) unit(x, y, h0[x], h1[x], rh, rl, ih, il);
       }

@@ -44,13 +44,27 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// (re, im) -> packed bf16 hi word and lo word (round to nearest even).
+// (re, im) -> packed bf16 hi word and lo word (round to nearest even).  The
+// residuals x - hi are exact in FP32; each is one v_dot2c_f32_bf16 of the hi
+// word with (-1, 0) or (0, -1) accumulated onto x, in place of unpacking hi
+// and subtracting (4 VALU operations per complex value instead of 6: the
+// staging VALU work, not the MFMAs, bounds these kernels' issue).
+// The (-1, 0) and (0, -1) bf16 pairs are moved into scalar registers by
+// hand: as an inline constant the compiler encodes (-1, 0) as "-1.0", which
+// this instruction reads as the f32 bit pattern, i.e. (0, -1).
+__device__ inline bf16x2 sreg_bf16x2(uint32_t bits) {
+  uint32_t r;
+  asm("s_mov_b32 %0, %1" : "=s"(r) : "n"(bits));
+  return __builtin_bit_cast(bf16x2, r);
+}
 __device__ inline void split_pair(float re, float im, uint32_t& hi,
                                   uint32_t& lo) {
   const f32x2 v = {re, im};
   const bf16x2 h = __builtin_convertvector(v, bf16x2);
-  const f32x2 hf = __builtin_convertvector(h, f32x2);
-  const f32x2 r = {v.x - hf.x, v.y - hf.y};
+  const bf16x2 mre = sreg_bf16x2(0x0000BF80u);   // (-1, 0)
+  const bf16x2 mim = sreg_bf16x2(0xBF800000u);   // (0, -1)
+  const f32x2 r = {__builtin_amdgcn_fdot2_f32_bf16(h, mre, re, false),
+                   __builtin_amdgcn_fdot2_f32_bf16(h, mim, im, false)};
   const bf16x2 l = __builtin_convertvector(r, bf16x2);
   hi = __builtin_bit_cast(uint32_t, h);
   lo = __builtin_bit_cast(uint32_t, l);
@@ -439,9 +453,10 @@ k_contract_bf16x3_batched(const KArgs<Idx>* __restrict__ inst,
 // when staged; tiles and stages that are wholly inside skip those selects.
 
 struct WaveRect { int r0, nr, c0, nc; };
+template <int R, int C> struct Rect { static constexpr int NR = R, NC = C; };
 constexpr WaveRect kRect144[8] = {
-    {0, 5, 0, 2}, {0, 5, 4, 2}, {5, 2, 0, 5}, {5, 4, 5, 3},
-    {0, 5, 2, 2}, {0, 5, 6, 2}, {7, 2, 0, 5}, {0, 9, 8, 1}};
+    {0, 5, 0, 2}, {0, 5, 2, 2}, {5, 4, 0, 3}, {5, 4, 3, 3},
+    {0, 5, 4, 2}, {0, 5, 6, 2}, {0, 9, 8, 1}, {5, 4, 6, 2}};
 
 constexpr int kP144 = 144;             // tile rows / columns
 constexpr int kPW = 16;                // words per LDS row (16 complex k)
@@ -453,6 +468,138 @@ __device__ inline int pword(int x, int w) {
   return x * kPW + ((((w >> 2) ^ (x >> 2)) & 3) << 2) + (w & 3);
 }
 
+// MFMA side of a 144 x 144 row-pair tile, shared by both staging paths:
+// wave (r0, c0)'s NR x NC units, their accumulators, the products of one
+// LDS stage, and the epilogue.  B's fragments (two forms of two planes, 16
+// registers a column) stay in registers for a k-chunk and A's stream, so the
+// second form of each B column is formed once per chunk (NC <= NR).
+template <typename Idx, int NR, int NC>
+struct PairUnits {
+  static_assert(NC <= NR, "B held");
+  f32x4 acc[NR][NC][2];
+  int r0, c0, g, l16;
+  uint32_t mim_r;
+
+  __device__ __forceinline__ void init(const KArgs<Idx>& p, int r0_, int c0_) {
+    r0 = r0_;
+    c0 = c0_;
+    const int lane = int(threadIdx.x) & 63;
+    g = lane >> 4;
+    l16 = lane & 15;
+    // B is stored in its re-form w ^ mre; the im-form is swap(w) ^ mim =
+    // swap(r) ^ swap(mre) ^ mim.
+    const uint32_t mre = (p.conj_a != 0) == (p.conj_b != 0) ? 0x80000000u : 0u;
+    const uint32_t mim = (p.conj_b ? 0x00008000u : 0u) | (p.conj_a ? 0x80000000u : 0u);
+    mim_r = mim ^ (mre ? 0x00008000u : 0u);
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        acc[r][c][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc[r][c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+  }
+  __device__ __forceinline__ u32x4 im_form(const u32x4& r) const {
+    u32x4 x;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) x[t] = __builtin_amdgcn_alignbit(r[t], r[t], 16) ^ mim_r;
+    return x;
+  }
+  __device__ __forceinline__ u32x4 frag(const uint32_t* pl, int row) const {
+    return *reinterpret_cast<const u32x4*>(pl + pword(row, 4 * g));
+  }
+  // Three products per unit and part: lo*hi, hi*lo, hi*hi.
+  __device__ __forceinline__ void unit(int r, int c, const u32x4& ah, const u32x4& al,
+                                       const u32x4& rh, const u32x4& rl, const u32x4& ih,
+                                       const u32x4& il) {
+    acc[r][c][0] = mfma_bf16(rl, ah, acc[r][c][0]);
+    acc[r][c][1] = mfma_bf16(il, ah, acc[r][c][1]);
+    acc[r][c][0] = mfma_bf16(rh, al, acc[r][c][0]);
+    acc[r][c][1] = mfma_bf16(ih, al, acc[r][c][1]);
+    acc[r][c][0] = mfma_bf16(rh, ah, acc[r][c][0]);
+    acc[r][c][1] = mfma_bf16(ih, ah, acc[r][c][1]);
+  }
+  // One k-chunk from the planes of an LDS stage (A hi, A lo, B re-hi, B re-lo).
+  __device__ __forceinline__ void compute(const uint32_t* stage) {
+    const uint32_t* pl0 = stage;
+    const uint32_t* pl1 = stage + kPPlane;
+    const uint32_t* pl2 = stage + 2 * kPPlane;
+    const uint32_t* pl3 = stage + 3 * kPPlane;
+    u32x4 h0[NC], h1[NC], h2[NC], h3[NC];
+#pragma unroll
+    for (int x = 0; x < NC; ++x) {
+      const int row = (c0 + x) * 16 + l16;
+      h0[x] = frag(pl2, row);
+      h1[x] = frag(pl3, row);
+      h2[x] = im_form(h0[x]);
+      h3[x] = im_form(h1[x]);
+    }
+#pragma unroll
+    for (int y = 0; y < NR; ++y) {
+      const int row = (r0 + y) * 16 + l16;
+      const u32x4 ah = frag(pl0, row), al = frag(pl1, row);
+#pragma unroll
+      for (int x = 0; x < NC; ++x) unit(y, x, ah, al, h0[x], h1[x], h2[x], h3[x]);
+    }
+  }
+  // Epilogue: lane holds C[i][j] (re, im) for i = unit row + l16 and
+  // j = unit column + 4 g + t.  C is read a row unit at a time (all of its
+  // columns) before any of it is written back.
+  __device__ __forceinline__ void store(const KArgs<Idx>& p, int tm, int tn) const {
+    const int64_t i0 = int64_t(tm) * kP144, j0 = int64_t(tn) * kP144;
+    float2* __restrict__ Cc = static_cast<float2*>(p.C);
+    const float ar = float(p.alpha_re), ai = float(p.alpha_im);
+    Idx cofs[NC][4];
+    bool cok[NC][4];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int64_t j = j0 + (c0 + c) * 16 + 4 * g + t;
+        cok[c][t] = j < p.N;
+        cofs[c][t] = cok[c][t] ? p.csC[j] : Idx(0);
+      }
+    const bool atomic = p.atomic_c != 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int64_t i = i0 + (r0 + r) * 16 + l16;
+      const bool rok = i < p.M;
+      const Idx ro = rok ? p.rsC[i] : Idx(0);
+      auto val = [&](int c, int t) {
+        const float re = acc[r][c][0][t], im = acc[r][c][1][t];
+        return float2{ar * re - ai * im, ar * im + ai * re};
+      };
+      if (atomic) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (rok && cok[c][t]) {
+              const float2 v = val(c, t);
+              float* d = reinterpret_cast<float*>(Cc + (ro + cofs[c][t]));
+              unsafeAtomicAdd(d, v.x);
+              unsafeAtomicAdd(d + 1, v.y);
+            }
+      } else {
+        float2 cv[NC][4];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) cv[c][t] = Cc[ro + cofs[c][t]];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (rok && cok[c][t]) {
+              const float2 v = val(c, t);
+              Cc[ro + cofs[c][t]] = float2{cv[c][t].x + v.x, cv[c][t].y + v.y};
+            }
+      }
+    }
+  }
+};
+
+// Staging with the k offsets read from the global maps (any offsets).
 template <typename Idx, bool AF, bool BF, int NR, int NC>
 __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* __restrict__ smem,
                                                  int tm, int tn, int kbeg, int kend, int r0,
@@ -462,7 +609,6 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
   auto plane = [&](int buf, int pl) { return smem + (buf * kPPlanes + pl) * kPPlane; };
 
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
   const int64_t M = p.M, N = p.N;
   const int K = kend;
@@ -479,22 +625,11 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
   ga.init(utid, i0, M, p.rsA);
   gb.init(utid, j0, N, p.csB);
   const bool a_inside = i0 + BM <= M, b_inside = j0 + BN <= N;
-  // B's words: re-form = w ^ mre, im-form = swap(w) ^ mim (see above; conj
-  // of A and of B flip the signs of the paired terms).
   const uint32_t mre = (p.conj_a != 0) == (p.conj_b != 0) ? 0x80000000u : 0u;
-  const uint32_t mim = (p.conj_b ? 0x00008000u : 0u) | (p.conj_a ? 0x80000000u : 0u);
-  // im-form from the stored re-form r = w ^ mre: swap(r) = swap(w) ^ swap(mre).
-  const uint32_t mim_r = mim ^ (mre ? 0x00008000u : 0u);
-  auto im_form = [&](const u32x4& r) {
-    u32x4 x;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) x[t] = __builtin_amdgcn_alignbit(r[t], r[t], 16) ^ mim_r;
-    return x;
-  };
 
   // Stage slot u of a gather (values v) into LDS: zero out-of-range entries
   // (edge tiles and the last stage only), split into hi / lo, store (A raw,
-  // B as its two forms).
+  // B in its re-form).
   auto put_quad = [&](const auto& gt, bool is_b, int buf, int p0, int u, bool edge,
                       const float2 (&v)[4]) {
     using G = std::remove_cv_t<std::remove_reference_t<decltype(gt)>>;
@@ -506,24 +641,13 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
       if (edge && !gt.inside(utid, p0, K, u, t)) re = im = 0.f;
       uint32_t h, l;
       split_pair(re, im, h, l);
-      hi[t] = h;
-      lo[t] = l;
+      hi[t] = is_b ? h ^ mre : h;
+      lo[t] = is_b ? l ^ mre : l;
     }
     if (G::valid(utid, u)) {
       const int w = pword(G::xc(e), 4 * G::kq(e));
-      if (is_b) {
-        u32x4 rh, rl;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          rh[t] = hi[t] ^ mre;
-          rl[t] = lo[t] ^ mre;
-        }
-        *reinterpret_cast<u32x4*>(plane(buf, 2) + w) = rh;
-        *reinterpret_cast<u32x4*>(plane(buf, 3) + w) = rl;
-      } else {
-        *reinterpret_cast<u32x4*>(plane(buf, 0) + w) = hi;
-        *reinterpret_cast<u32x4*>(plane(buf, 1) + w) = lo;
-      }
+      *reinterpret_cast<u32x4*>(plane(buf, is_b ? 2 : 0) + w) = hi;
+      *reinterpret_cast<u32x4*>(plane(buf, is_b ? 3 : 1) + w) = lo;
     }
   };
   auto put_word = [&](const auto& gt, bool is_b, int buf, int p0, int u, bool edge, float2 v) {
@@ -534,13 +658,8 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
     split_pair(v.x, v.y, hi, lo);
     if (G::valid(utid, u)) {
       const int w = pword(G::xc(e), G::kc(e));
-      if (is_b) {
-        plane(buf, 2)[w] = hi ^ mre;
-        plane(buf, 3)[w] = lo ^ mre;
-      } else {
-        plane(buf, 0)[w] = hi;
-        plane(buf, 1)[w] = lo;
-      }
+      plane(buf, is_b ? 2 : 0)[w] = is_b ? hi ^ mre : hi;
+      plane(buf, is_b ? 3 : 1)[w] = is_b ? lo ^ mre : lo;
     }
   };
   // Gathered values of one stage, between their request and their staging.
@@ -566,75 +685,12 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
     }
   };
 
-  f32x4 acc[NR][NC][2];
-#pragma unroll
-  for (int r = 0; r < NR; ++r)
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      acc[r][c][0] = f32x4{0.f, 0.f, 0.f, 0.f};
-      acc[r][c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-    }
-
-  const int g = lane >> 4;
-  const int l16 = lane & 15;
-  auto frag = [&](const uint32_t* pl, int row) {
-    return *reinterpret_cast<const u32x4*>(pl + pword(row, 4 * g));
-  };
-  // Three products per unit and part: lo*hi, hi*lo, hi*hi.
-  auto unit = [&](int r, int c, const u32x4& ah, const u32x4& al, const u32x4& rh,
-                  const u32x4& rl, const u32x4& ih, const u32x4& il) {
-    acc[r][c][0] = mfma_bf16(rl, ah, acc[r][c][0]);
-    acc[r][c][1] = mfma_bf16(il, ah, acc[r][c][1]);
-    acc[r][c][0] = mfma_bf16(rh, al, acc[r][c][0]);
-    acc[r][c][1] = mfma_bf16(ih, al, acc[r][c][1]);
-    acc[r][c][0] = mfma_bf16(rh, ah, acc[r][c][0]);
-    acc[r][c][1] = mfma_bf16(ih, ah, acc[r][c][1]);
-  };
-  // One k-chunk: the fragments of one dimension stay in registers -- B's
-  // (two forms of two planes, 16 registers a column) when they take no more
-  // registers than A's (two planes, 8 a row).
-  constexpr bool HOLD_B = 2 * NC <= NR;
-  constexpr int NS = HOLD_B ? NR : NC;     // streamed
-  constexpr int NH = HOLD_B ? NC : NR;     // held
-  auto compute = [&](int buf) {
-    u32x4 h0[NH], h1[NH], h2[NH], h3[NH];
-#pragma unroll
-    for (int x = 0; x < NH; ++x) {
-      if constexpr (HOLD_B) {
-        const int row = (c0 + x) * 16 + l16;
-        h0[x] = frag(plane(buf, 2), row);
-        h1[x] = frag(plane(buf, 3), row);
-        h2[x] = im_form(h0[x]);
-        h3[x] = im_form(h1[x]);
-      } else {
-        const int row = (r0 + x) * 16 + l16;
-        h0[x] = frag(plane(buf, 0), row);
-        h1[x] = frag(plane(buf, 1), row);
-      }
-    }
-#pragma unroll
-    for (int y = 0; y < NS; ++y) {
-      if constexpr (HOLD_B) {
-        const int row = (r0 + y) * 16 + l16;
-        const u32x4 ah = frag(plane(buf, 0), row), al = frag(plane(buf, 1), row);
-#pragma unroll
-        for (int x = 0; x < NH; ++x) unit(y, x, ah, al, h0[x], h1[x], h2[x], h3[x]);
-      } else {
-        const int row = (c0 + y) * 16 + l16;
-        const u32x4 rh = frag(plane(buf, 2), row), rl = frag(plane(buf, 3), row);
-        const u32x4 ih = im_form(rh), il = im_form(rl);
-#pragma unroll
-        for (int x = 0; x < NH; ++x) unit(x, y, h0[x], h1[x], rh, rl, ih, il);
-      }
-    }
-  };
+  PairUnits<Idx, NR, NC> mm;
+  mm.init(p, r0, c0);
 
   // Pipeline (LDS buffers alternate by stage): stage s + 1 is requested at
   // the top of stage s and staged after its products.  Requests past the
-  // last stage read clamped, valid offsets.  (Measured without gain: a
-  // second register set requesting two stages ahead (spills), staging
-  // between the halves of the products, and skewing the two waves of a
-  // SIMD so that one stages while the other multiplies.)
+  // last stage read clamped, valid offsets.
   const int nst = (K - kbeg + BK - 1) / BK;
   ga.load_k(utid, kbeg, K, p.csA);
   gb.load_k(utid, kbeg, K, p.rsB);
@@ -647,106 +703,280 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
     load_stage();
     ga.load_k(utid, kbeg + (s + 2) * BK, K, p.csA);
     gb.load_k(utid, kbeg + (s + 2) * BK, K, p.rsB);
-    compute(s & 1);
+    mm.compute(plane(s & 1, 0));
     swrite((s + 1) & 1, kbeg + (s + 1) * BK);
     __syncthreads();
   }
-  compute((nst - 1) & 1);
-
-  // Epilogue: lane holds C[i][j] (re, im) for i = unit row + l16 and
-  // j = unit column + 4 g + t.  C is read a row unit at a time (all of its
-  // columns) before any of it is written back.
-  float2* __restrict__ Cc = static_cast<float2*>(p.C);
-  const float ar = float(p.alpha_re), ai = float(p.alpha_im);
-  Idx cofs[NC][4];
-  bool cok[NC][4];
-#pragma unroll
-  for (int c = 0; c < NC; ++c)
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int64_t j = j0 + (c0 + c) * 16 + 4 * g + t;
-      cok[c][t] = j < N;
-      cofs[c][t] = cok[c][t] ? p.csC[j] : Idx(0);
-    }
-  const bool atomic = p.atomic_c != 0;
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    const int64_t i = i0 + (r0 + r) * 16 + l16;
-    const bool rok = i < M;
-    const Idx ro = rok ? p.rsC[i] : Idx(0);
-    auto val = [&](int c, int t) {
-      const float re = acc[r][c][0][t], im = acc[r][c][1][t];
-      return float2{ar * re - ai * im, ar * im + ai * re};
-    };
-    if (atomic) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-          if (rok && cok[c][t]) {
-            const float2 v = val(c, t);
-            float* d = reinterpret_cast<float*>(Cc + (ro + cofs[c][t]));
-            unsafeAtomicAdd(d, v.x);
-            unsafeAtomicAdd(d + 1, v.y);
-          }
-    } else {
-      float2 cv[NC][4];
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) cv[c][t] = Cc[ro + cofs[c][t]];
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-          if (rok && cok[c][t]) {
-            const float2 v = val(c, t);
-            Cc[ro + cofs[c][t]] = float2{cv[c][t].x + v.x, cv[c][t].y + v.y};
-          }
-    }
-  }
+  mm.compute(plane((nst - 1) & 1, 0));
+  mm.store(p, tm, tn);
 }
 
-template <typename Idx, bool AF, bool BF>
+// Gather slots of one operand for the LDS-map path: offsets are 32-bit
+// byte offsets from the operand's rebased base.  QUAD: lanes along the free
+// index x, a slot holds the four consecutive k = kc .. kc + 3 of one row
+// (one 16-byte LDS write per plane).  Otherwise lanes along k: a slot holds
+// one element of row e / 16 at k = tid % 16 (the same k for every slot).
+template <bool QUAD>
+struct LdsGather {
+  static constexpr int NT = 512;
+  static constexpr int TOTAL = QUAD ? kP144 * 4 : kP144 * 16;
+  static constexpr int E = (TOTAL + NT - 1) / NT;
+  static constexpr bool RAGGED = (TOTAL % NT) != 0;
+  uint32_t xb[E];   // byte offset of the slot's row
+  uint32_t xok;     // bit u: slot u's row lies inside the problem
+
+  __device__ static bool valid(unsigned tid, int u) {
+    return !RAGGED || u + 1 < E || tid + unsigned(NT) * u < unsigned(TOTAL);
+  }
+  __device__ static unsigned elem(unsigned tid, int u) {
+    return valid(tid, u) ? tid + unsigned(NT) * u : 0u;
+  }
+  __device__ static int xc(unsigned e) { return QUAD ? int(e % unsigned(kP144)) : int(e / 16u); }
+  // First k of slot u within a stage.
+  __device__ static int kc(unsigned tid, int u) {
+    return QUAD ? 4 * int(elem(tid, u) / unsigned(kP144)) : int(tid & 15u);
+  }
+  __device__ void init(unsigned tid, int64_t x0, int64_t X, const int32_t* xmap, int64_t xmin) {
+    xok = 0;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int64_t xg = x0 + xc(elem(tid, u));
+      xb[u] = uint32_t(int64_t(xmap[xg < X ? xg : X - 1]) - xmin) * 8u;
+      xok |= ((xg < X && valid(tid, u)) ? 1u : 0u) << u;
+    }
+  }
+};
+
+// k-map entries per operand staged in LDS per launch (the launcher splits
+// longer contracted ranges into sequential launches).
+constexpr int kKMapMax = 8192;
+
+// Staging with the contracted-index maps in LDS: each workgroup copies the
+// k offsets of its range once, as 32-bit byte offsets from the operands'
+// rebased bases (every offset is non-negative and the operands' byte spans
+// are below 2^32), so a gather address is one 32-bit add on a scalar base
+// and the per-stage k offsets are LDS reads -- far fewer VALU operations
+// than 64-bit address arithmetic and per-stage map loads, and the VALU
+// issue, not the MFMAs, bounds this kernel.  Tiles wholly inside M and N run
+// their stages wholly inside K in a loop without the zeroing selects.
+template <bool AF, bool BF, int NR, int NC>
+__device__ __forceinline__ void bf16x3_pair_tile_lds(const KArgs<int32_t>& p,
+                                                     uint32_t* __restrict__ smem,
+                                                     uint32_t* __restrict__ kmap, int tm, int tn,
+                                                     int kbeg, int kend, int r0, int c0) {
+  constexpr int BM = kP144, BN = kP144, BK = 16;
+  auto plane = [&](int buf, int pl) { return smem + (buf * kPPlanes + pl) * kPPlane; };
+  const unsigned tid = threadIdx.x;
+  const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
+  const int K = kend;
+  const int nst = (K - kbeg + BK - 1) / BK;
+  uint32_t* __restrict__ kmA = kmap;
+  uint32_t* __restrict__ kmB = kmap + kKMapMax;
+  for (int i = int(tid); i < nst * BK; i += 512) {
+    const int k = kbeg + i < K ? kbeg + i : K - 1;
+    kmA[i] = uint32_t(int64_t(p.csA[k]) - p.min_csA) * 8u;
+    kmB[i] = uint32_t(int64_t(p.rsB[k]) - p.min_rsB) * 8u;
+  }
+  const char* __restrict__ Ab = static_cast<const char*>(p.A) + 8 * (p.min_rsA + p.min_csA);
+  const char* __restrict__ Bb = static_cast<const char*>(p.B) + 8 * (p.min_rsB + p.min_csB);
+  LdsGather<AF> ga;
+  LdsGather<BF> gb;
+  ga.init(tid, i0, p.M, p.rsA, p.min_rsA);
+  gb.init(tid, j0, p.N, p.csB, p.min_csB);
+  const bool inside = i0 + BM <= p.M && j0 + BN <= p.N;
+  const uint32_t mre = (p.conj_a != 0) == (p.conj_b != 0) ? 0x80000000u : 0u;
+
+  // k offsets of one stage (registers), read from LDS one stage ahead.
+  using KA = std::conditional_t<AF, u32x4[LdsGather<AF>::E], uint32_t>;
+  using KB = std::conditional_t<BF, u32x4[LdsGather<BF>::E], uint32_t>;
+  KA ka;
+  KB kb;
+  auto kread = [&](const uint32_t* km, auto& kx, const auto& gt, int t) {
+    using G = std::remove_cv_t<std::remove_reference_t<decltype(gt)>>;
+    const int tt = t < nst ? t : nst - 1;
+    if constexpr (std::is_same_v<std::remove_reference_t<decltype(kx)>, uint32_t>) {
+      kx = km[tt * BK + G::kc(tid, 0)];
+    } else {
+#pragma unroll
+      for (int u = 0; u < G::E; ++u)
+        kx[u] = *reinterpret_cast<const u32x4*>(km + tt * BK + G::kc(tid, u));
+    }
+  };
+  using VA = std::conditional_t<AF, float2[LdsGather<AF>::E][4], float2[LdsGather<AF>::E]>;
+  using VB = std::conditional_t<BF, float2[LdsGather<BF>::E][4], float2[LdsGather<BF>::E]>;
+  VA va;
+  VB vb;
+  auto request = [&](const char* base, const auto& gt, const auto& kx, auto& v) {
+    using G = std::remove_cv_t<std::remove_reference_t<decltype(gt)>>;
+#pragma unroll
+    for (int u = 0; u < G::E; ++u) {
+      if constexpr (std::is_same_v<std::remove_cv_t<std::remove_reference_t<decltype(kx)>>,
+                                   uint32_t>) {
+        v[u] = *reinterpret_cast<const float2*>(base + uint64_t(gt.xb[u] + kx));
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          v[u][j] = *reinterpret_cast<const float2*>(base + uint64_t(gt.xb[u] + kx[u][j]));
+      }
+    }
+  };
+  // Stage t's values into LDS buffer buf; EDGE zeroes entries outside the
+  // problem (rows of edge tiles, k past the end).
+  auto put = [&](const auto& gt, bool is_b, auto edge_tag, int buf, int t, auto& v) {
+    using G = std::remove_cv_t<std::remove_reference_t<decltype(gt)>>;
+    constexpr bool EDGE = decltype(edge_tag)::value;
+    const int kst = kbeg + t * BK;   // first k of the stage
+    uint32_t* ph = plane(buf, is_b ? 2 : 0);
+    uint32_t* pl = plane(buf, is_b ? 3 : 1);
+    const uint32_t m = is_b ? mre : 0u;
+#pragma unroll
+    for (int u = 0; u < G::E; ++u) {
+      const unsigned e = G::elem(tid, u);
+      const int x = G::xc(e), k0 = G::kc(tid, u);
+      const bool rok = (gt.xok >> u) & 1u;
+      if constexpr (std::is_same_v<std::remove_cv_t<std::remove_reference_t<decltype(v[0])>>,
+                                   float2>) {
+        float2 w = v[u];
+        if (EDGE && !(rok && kst + k0 < K)) w = float2{0.f, 0.f};
+        uint32_t h, l;
+        split_pair(w.x, w.y, h, l);
+        if (G::valid(tid, u)) {
+          const int a = pword(x, k0);
+          ph[a] = h ^ m;
+          pl[a] = l ^ m;
+        }
+      } else {
+        u32x4 hi, lo;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          float2 w = v[u][j];
+          if (EDGE && !(rok && kst + k0 + j < K)) w = float2{0.f, 0.f};
+          uint32_t h, l;
+          split_pair(w.x, w.y, h, l);
+          hi[j] = h ^ m;
+          lo[j] = l ^ m;
+        }
+        if (G::valid(tid, u)) {
+          const int a = pword(x, k0);
+          *reinterpret_cast<u32x4*>(ph + a) = hi;
+          *reinterpret_cast<u32x4*>(pl + a) = lo;
+        }
+      }
+    }
+  };
+  auto swrite = [&](auto edge_tag, int buf, int t) {
+    put(ga, false, edge_tag, buf, t, va);
+    put(gb, true, edge_tag, buf, t, vb);
+  };
+
+  PairUnits<int32_t, NR, NC> mm;
+  mm.init(p, r0, c0);
+
+  __syncthreads();   // the k maps
+  kread(kmA, ka, ga, 0);
+  kread(kmB, kb, gb, 0);
+  request(Ab, ga, ka, va);
+  request(Bb, gb, kb, vb);
+  kread(kmA, ka, ga, 1);
+  kread(kmB, kb, gb, 1);
+  swrite(std::true_type{}, 0, 0);
+  __syncthreads();
+  // Iteration s: request stage s + 1 (offsets read last iteration), read
+  // the offsets of stage s + 2, multiply stage s, stage s + 1.
+  // With both operands gathered along k, a scheduling barrier keeps the
+  // requests at the top of the stage: the compiler otherwise sinks them
+  // into the products and their latency shows when the stage is staged
+  // (48^4 bdac,dbef: 360 -> 310 us; the same barrier costs the layouts with
+  // a quad gather 4-10 %).
+  auto body = [&](auto edge_tag, int s) {
+    request(Ab, ga, ka, va);
+    request(Bb, gb, kb, vb);
+    kread(kmA, ka, ga, s + 2);
+    kread(kmB, kb, gb, s + 2);
+    if constexpr (!AF && !BF) __builtin_amdgcn_sched_barrier(0);
+    mm.compute(plane(s & 1, 0));
+    swrite(edge_tag, (s + 1) & 1, s + 1);
+    __syncthreads();
+  };
+  const int nfull = (K - kbeg) / BK;   // stages 0 .. nfull - 1 lie wholly inside K
+  int s = 0;
+  if (inside)
+    for (; s + 1 < nst && s + 1 < nfull; ++s) body(std::false_type{}, s);
+  for (; s + 1 < nst; ++s) body(std::true_type{}, s);
+  mm.compute(plane((nst - 1) & 1, 0));
+  mm.store(p, tm, tn);
+}
+
+template <typename Idx, bool AF, bool BF, bool LDSMAP>
 __global__ void __launch_bounds__(512) k_contract_bf16x3_pair(KArgs<Idx> p) {
   __shared__ __attribute__((aligned(16))) uint32_t smem[2 * kPPlanes * kPPlane];
+  __shared__ __attribute__((aligned(16))) uint32_t kmap[LDSMAP ? 2 * kKMapMax : 4];
   int tm, tn, kbeg, kend;
   tile_and_krange(p, tm, tn, kbeg, kend);
   const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
   const int r0 = kRect144[wid & 7].r0, c0 = kRect144[wid & 7].c0;
+  auto run = [&](auto rect) {
+    constexpr int NR = decltype(rect)::NR, NC = decltype(rect)::NC;
+    if constexpr (LDSMAP)
+      bf16x3_pair_tile_lds<AF, BF, NR, NC>(p, smem, kmap, tm, tn, kbeg, kend, r0, c0);
+    else
+      bf16x3_pair_tile<Idx, AF, BF, NR, NC>(p, smem, tm, tn, kbeg, kend, r0, c0);
+  };
   switch (wid) {
     case 2:
-    case 6:
-      bf16x3_pair_tile<Idx, AF, BF, 2, 5>(p, smem, tm, tn, kbeg, kend, r0, c0);
-      break;
     case 3:
-      bf16x3_pair_tile<Idx, AF, BF, 4, 3>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      run(Rect<4, 3>{});
+      break;
+    case 6:
+      run(Rect<9, 1>{});
       break;
     case 7:
-      bf16x3_pair_tile<Idx, AF, BF, 9, 1>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      run(Rect<4, 2>{});
       break;
     default:
-      bf16x3_pair_tile<Idx, AF, BF, 5, 2>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      run(Rect<5, 2>{});
       break;
   }
 }
 
-template <typename Idx>
-hipError_t launch_pair144(const GemmProblem& g, hipStream_t s) {
+template <typename Idx, bool LDSMAP>
+hipError_t launch_pair144_as(const GemmProblem& g, hipStream_t s) {
   const KArgs<Idx> a = make_args<Idx>(g, kP144, kP144, 16);
   const int64_t per = int64_t(a.tiles_m) * a.tiles_n * a.ksplit;
   const dim3 block(512);
   const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
   return launch_batches(a, per, 512, [&](const KArgs<Idx>& q, unsigned grid) {
     if (af && bf)
-      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, true, true>), grid, block, 0, s, q);
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, true, true, LDSMAP>), grid, block, 0, s, q);
     else if (af)
-      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, true, false>), grid, block, 0, s, q);
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, true, false, LDSMAP>), grid, block, 0, s, q);
     else if (bf)
-      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, false, true>), grid, block, 0, s, q);
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, false, true, LDSMAP>), grid, block, 0, s, q);
     else
-      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, false, false>), grid, block, 0, s, q);
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, false, false, LDSMAP>), grid, block, 0, s, q);
   });
+}
+
+// 32-bit offsets whose operands span less than 2^32 bytes take the LDS-map
+// kernel, in sequential launches of at most kKMapMax contracted indices
+// (each adds its partial products into C; launches on one stream run in
+// order).
+template <typename Idx>
+hipError_t launch_pair144(const GemmProblem& g, hipStream_t s) {
+  if constexpr (std::is_same_v<Idx, int32_t>) {
+    if (g.span32) {
+      for (int64_t k0 = 0; k0 < g.K; k0 += kKMapMax) {
+        GemmProblem h = g;
+        h.K = g.K - k0 < kKMapMax ? g.K - k0 : kKMapMax;
+        h.csA = static_cast<const int32_t*>(g.csA) + k0;
+        h.rsB = static_cast<const int32_t*>(g.rsB) + k0;
+        const hipError_t e = launch_pair144_as<int32_t, true>(h, s);
+        if (e != hipSuccess) return e;
+      }
+      return hipSuccess;
+    }
+  }
+  return launch_pair144_as<Idx, false>(g, s);
 }
 
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN>

@@ -42,6 +42,8 @@ struct KArgs {
   const int64_t* boffA;   // byte offsets of the batch problems (nbatch > 1)
   const int64_t* boffB;
   const int64_t* boffC;
+  int64_t min_rsA, min_csA, min_rsB, min_csB;   // smallest map entries (<= 0)
+  int32_t span32;   // A and B span < 2^32 bytes from their rebased bases
 };
 
 // Moves the operand pointers of q to batch problem z (of this launch).
@@ -398,6 +400,9 @@ KArgs<Idx> make_args(const GemmProblem& g, int bm, int bn, int bk = 16) {
   a.boffA = g.L > 1 ? g.boffA : nullptr;
   a.boffB = g.L > 1 ? g.boffB : nullptr;
   a.boffC = g.L > 1 ? g.boffC : nullptr;
+  a.min_rsA = g.min_rsA; a.min_csA = g.min_csA;
+  a.min_rsB = g.min_rsB; a.min_csB = g.min_csB;
+  a.span32 = g.span32;
   const int64_t tiles = int64_t(a.tiles_m) * a.tiles_n * (g.L > 0 ? g.L : 1);
   const int cus = g.cus > 0 ? g.cus : 256;
   if (g.split_k == 0 && bm > 0 && tiles > 0 && 2 * tiles < cus) {

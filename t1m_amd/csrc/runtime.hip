@@ -531,6 +531,16 @@ int64_t max_abs_offset(const std::vector<Mode>& modes, int which) {
   return s;
 }
 
+// Smallest element offset of a bundle's scatter vector (<= 0).
+int64_t min_offset(const std::vector<Mode>& modes, int which) {
+  int64_t s = 0;
+  for (const Mode& m : modes) {
+    const int64_t st = which == 0 ? m.sa : which == 1 ? m.sb : m.sc;
+    if (st < 0) s += (m.dim > 0 ? m.dim - 1 : 0) * st;
+  }
+  return s;
+}
+
 int64_t min_stride(const std::vector<Mode>& modes, int which) {
   int64_t best = INT64_MAX;
   for (const Mode& m : modes) {
@@ -630,6 +640,9 @@ void transpose_problem(GemmProblem& g) {
   g.b_kfast = amf ? 0 : 1;               // A gathered along k <=> A^T along k
   std::swap(g.boffA, g.boffB);
   std::swap(g.a_pairs, g.b_pairs);       // a property of each tensor's gather map
+  const int64_t mra = g.min_rsA, mca = g.min_csA;
+  g.min_rsA = g.min_csB; g.min_csA = g.min_rsB;
+  g.min_rsB = mca; g.min_csB = mra;
 }
 
 // True unless the layout provably gives every index tuple its own element:
@@ -707,6 +720,15 @@ void plan_layout(const Problem& P0, int precision, Plan& pl, Problem& P) {
   pl.idx32 = (offA <= lim && offB <= lim && offC <= lim) ? 1 : 0;
   if (const char* f = env_override("T1M_FORCE_IDX64"))
     if (atoi(f) != 0) pl.idx32 = 0;
+  pl.min_rsA = min_offset(P.m, 0);
+  pl.min_csA = min_offset(P.k, 0);
+  pl.min_rsB = min_offset(P.k, 1);
+  pl.min_csB = min_offset(P.n, 1);
+  // A map entry minus its map's minimum is at most the map's span, so a
+  // row + k byte offset is at most (offX + 1) elements: below 2^32 bytes.
+  const int64_t esz = dtype_size(P.dtype);
+  const int64_t lim32 = int64_t(1) << 32;
+  pl.span32 = ((offA + 1) * esz < lim32 && (offB + 1) * esz < lim32) ? 1 : 0;
   pl.modes_m = P.m; pl.modes_n = P.n; pl.modes_k = P.k;
   // Two-element gathers (complex<float>): the leading mode (extent > 1) of
   // the operand's gather direction has unit stride and even extent, so
@@ -1108,6 +1130,9 @@ GemmProblem gemm_for(const Plan& pl, const void* A, const void* B, void* C,
   g.a_mfast = pl.a_mfast; g.b_kfast = pl.b_kfast;
   g.a_pairs = pl.a_pairs; g.b_pairs = pl.b_pairs;
   g.idx32 = pl.idx32;
+  g.min_rsA = pl.min_rsA; g.min_csA = pl.min_csA;
+  g.min_rsB = pl.min_rsB; g.min_csB = pl.min_csB;
+  g.span32 = pl.idx32 && pl.span32 ? 1 : 0;
   g.dtype = pl.dtype;
   g.split_k = opt.deterministic ? 1 : 0;
   g.cus = cu_count();

@@ -453,7 +453,6 @@ k_contract_bf16x3_batched(const KArgs<Idx>* __restrict__ inst,
 // when staged; tiles and stages that are wholly inside skip those selects.
 
 struct WaveRect { int r0, nr, c0, nc; };
-template <int R, int C> struct Rect { static constexpr int NR = R, NC = C; };
 constexpr WaveRect kRect144[8] = {
     {0, 5, 0, 2}, {0, 5, 2, 2}, {5, 4, 0, 3}, {5, 4, 3, 3},
     {0, 5, 4, 2}, {0, 5, 6, 2}, {0, 9, 8, 1}, {5, 4, 6, 2}};
@@ -468,14 +467,16 @@ __device__ inline int pword(int x, int w) {
   return x * kPW + ((((w >> 2) ^ (x >> 2)) & 3) << 2) + (w & 3);
 }
 
-// MFMA side of a 144 x 144 row-pair tile, shared by both staging paths:
+// MFMA side of a 144 x 144 row-pair tile, shared by the staging paths:
 // wave (r0, c0)'s NR x NC units, their accumulators, the products of one
-// LDS stage, and the epilogue.  B's fragments (two forms of two planes, 16
-// registers a column) stay in registers for a k-chunk and A's stream, so the
-// second form of each B column is formed once per chunk (NC <= NR).
+// LDS stage, and the epilogue.  Per k-chunk every fragment is read from LDS
+// once (and every B column's second form made once); the fragments of one
+// dimension stay in registers while the other's stream -- B's (two forms of
+// two planes, 16 registers a column) when they take no more registers than
+// A's (two planes, 8 a row).
 template <typename Idx, int NR, int NC>
 struct PairUnits {
-  static_assert(NC <= NR, "B held");
+  static constexpr bool HOLD_B = 2 * NC <= NR;
   f32x4 acc[NR][NC][2];
   int r0, c0, g, l16;
   uint32_t mim_r;
@@ -525,31 +526,61 @@ struct PairUnits {
     const uint32_t* pl1 = stage + kPPlane;
     const uint32_t* pl2 = stage + 2 * kPPlane;
     const uint32_t* pl3 = stage + 3 * kPPlane;
-    u32x4 h0[NC], h1[NC], h2[NC], h3[NC];
+    if constexpr (HOLD_B) {
+      u32x4 h0[NC], h1[NC], h2[NC], h3[NC];
 #pragma unroll
-    for (int x = 0; x < NC; ++x) {
-      const int row = (c0 + x) * 16 + l16;
-      h0[x] = frag(pl2, row);
-      h1[x] = frag(pl3, row);
-      h2[x] = im_form(h0[x]);
-      h3[x] = im_form(h1[x]);
-    }
+      for (int x = 0; x < NC; ++x) {
+        const int row = (c0 + x) * 16 + l16;
+        h0[x] = frag(pl2, row);
+        h1[x] = frag(pl3, row);
+        h2[x] = im_form(h0[x]);
+        h3[x] = im_form(h1[x]);
+      }
 #pragma unroll
-    for (int y = 0; y < NR; ++y) {
-      const int row = (r0 + y) * 16 + l16;
-      const u32x4 ah = frag(pl0, row), al = frag(pl1, row);
+      for (int y = 0; y < NR; ++y) {
+        const int row = (r0 + y) * 16 + l16;
+        const u32x4 ah = frag(pl0, row), al = frag(pl1, row);
 #pragma unroll
-      for (int x = 0; x < NC; ++x) unit(y, x, ah, al, h0[x], h1[x], h2[x], h3[x]);
+        for (int x = 0; x < NC; ++x) unit(y, x, ah, al, h0[x], h1[x], h2[x], h3[x]);
+      }
+    } else {
+      u32x4 h0[NR], h1[NR];
+#pragma unroll
+      for (int y = 0; y < NR; ++y) {
+        const int row = (r0 + y) * 16 + l16;
+        h0[y] = frag(pl0, row);
+        h1[y] = frag(pl1, row);
+      }
+#pragma unroll
+      for (int x = 0; x < NC; ++x) {
+        const int row = (c0 + x) * 16 + l16;
+        const u32x4 rh = frag(pl2, row), rl = frag(pl3, row);
+        const u32x4 ih = im_form(rh), il = im_form(rl);
+#pragma unroll
+        for (int y = 0; y < NR; ++y) unit(y, x, h0[y], h1[y], rh, rl, ih, il);
+      }
     }
   }
   // Epilogue: lane holds C[i][j] (re, im) for i = unit row + l16 and
   // j = unit column + 4 g + t.  C is read a row unit at a time (all of its
-  // columns) before any of it is written back.
+  // columns) before any of it is written back.  OFF32 (span32 plans):
+  // 32-bit byte offsets from C's rebased base, one register an address.
+  template <bool OFF32 = false>
   __device__ __forceinline__ void store(const KArgs<Idx>& p, int tm, int tn) const {
     const int64_t i0 = int64_t(tm) * kP144, j0 = int64_t(tn) * kP144;
-    float2* __restrict__ Cc = static_cast<float2*>(p.C);
+    using Off = std::conditional_t<OFF32, uint32_t, Idx>;
+    char* __restrict__ Cb = static_cast<char*>(p.C);
+    if constexpr (OFF32) Cb += 8 * (p.min_rsC + p.min_csC);
+    auto off = [&](const Idx* map, int64_t x, int64_t mn) -> Off {
+      if constexpr (OFF32) return uint32_t(int64_t(map[x]) - mn) * 8u;
+      else return map[x];
+    };
+    auto at = [&](Off ro, Off co) -> float2* {
+      if constexpr (OFF32) return reinterpret_cast<float2*>(Cb + uint64_t(ro + co));
+      else return reinterpret_cast<float2*>(Cb) + (int64_t(ro) + int64_t(co));
+    };
     const float ar = float(p.alpha_re), ai = float(p.alpha_im);
-    Idx cofs[NC][4];
+    Off cofs[NC][4];
     bool cok[NC][4];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -557,14 +588,14 @@ struct PairUnits {
       for (int t = 0; t < 4; ++t) {
         const int64_t j = j0 + (c0 + c) * 16 + 4 * g + t;
         cok[c][t] = j < p.N;
-        cofs[c][t] = cok[c][t] ? p.csC[j] : Idx(0);
+        cofs[c][t] = off(p.csC, cok[c][t] ? j : p.N - 1, p.min_csC);
       }
     const bool atomic = p.atomic_c != 0;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       const int64_t i = i0 + (r0 + r) * 16 + l16;
       const bool rok = i < p.M;
-      const Idx ro = rok ? p.rsC[i] : Idx(0);
+      const Off ro = off(p.rsC, rok ? i : p.M - 1, p.min_rsC);
       auto val = [&](int c, int t) {
         const float re = acc[r][c][0][t], im = acc[r][c][1][t];
         return float2{ar * re - ai * im, ar * im + ai * re};
@@ -576,7 +607,7 @@ struct PairUnits {
           for (int t = 0; t < 4; ++t)
             if (rok && cok[c][t]) {
               const float2 v = val(c, t);
-              float* d = reinterpret_cast<float*>(Cc + (ro + cofs[c][t]));
+              float* d = reinterpret_cast<float*>(at(ro, cofs[c][t]));
               unsafeAtomicAdd(d, v.x);
               unsafeAtomicAdd(d + 1, v.y);
             }
@@ -585,14 +616,14 @@ struct PairUnits {
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-          for (int t = 0; t < 4; ++t) cv[c][t] = Cc[ro + cofs[c][t]];
+          for (int t = 0; t < 4; ++t) cv[c][t] = *at(ro, cofs[c][t]);
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
           for (int t = 0; t < 4; ++t)
             if (rok && cok[c][t]) {
               const float2 v = val(c, t);
-              Cc[ro + cofs[c][t]] = float2{cv[c][t].x + v.x, cv[c][t].y + v.y};
+              *at(ro, cofs[c][t]) = float2{cv[c][t].x + v.x, cv[c][t].y + v.y};
             }
       }
     }
@@ -716,9 +747,8 @@ __device__ __forceinline__ void bf16x3_pair_tile(const KArgs<Idx>& p, uint32_t* 
 // index x, a slot holds the four consecutive k = kc .. kc + 3 of one row
 // (one 16-byte LDS write per plane).  Otherwise lanes along k: a slot holds
 // one element of row e / 16 at k = tid % 16 (the same k for every slot).
-template <bool QUAD>
+template <bool QUAD, int NT = 512>
 struct LdsGather {
-  static constexpr int NT = 512;
   static constexpr int TOTAL = QUAD ? kP144 * 4 : kP144 * 16;
   static constexpr int E = (TOTAL + NT - 1) / NT;
   static constexpr bool RAGGED = (TOTAL % NT) != 0;
@@ -751,88 +781,80 @@ struct LdsGather {
 // longer contracted ranges into sequential launches).
 constexpr int kKMapMax = 8192;
 
-// Staging with the contracted-index maps in LDS: each workgroup copies the
-// k offsets of its range once, as 32-bit byte offsets from the operands'
-// rebased bases (every offset is non-negative and the operands' byte spans
-// are below 2^32), so a gather address is one 32-bit add on a scalar base
-// and the per-stage k offsets are LDS reads -- far fewer VALU operations
-// than 64-bit address arithmetic and per-stage map loads, and the VALU
-// issue, not the MFMAs, bounds this kernel.  Tiles wholly inside M and N run
-// their stages wholly inside K in a loop without the zeroing selects.
-template <bool AF, bool BF, int NR, int NC>
-__device__ __forceinline__ void bf16x3_pair_tile_lds(const KArgs<int32_t>& p,
-                                                     uint32_t* __restrict__ smem,
-                                                     uint32_t* __restrict__ kmap, int tm, int tn,
-                                                     int kbeg, int kend, int r0, int c0) {
-  constexpr int BM = kP144, BN = kP144, BK = 16;
+// Dedicated staging waves for the LDS-map path: kPairProducers waves after
+// the kPairConsumers MFMA waves gather, split and stage every stage of A and
+// B, two stages ahead of the products (stage s + 1 is staged while the MFMA
+// waves multiply stage s; the loads of stage s + 2 are in flight
+// meanwhile).  With the staging's VALU work cut down to its core (32-bit
+// addresses, LDS k offsets, the dot2 split), the two roles overlap on each
+// SIMD instead of alternating phases at every barrier.
+constexpr int kPairConsumers = 8;
+constexpr int kPairProducers = 4;
+constexpr int kPairWsThreads = 64 * (kPairConsumers + kPairProducers);
+
+template <bool AF, bool BF>
+__device__ __forceinline__ void pair_ws_producer(const KArgs<int32_t>& p,
+                                                 uint32_t* __restrict__ smem,
+                                                 const uint32_t* __restrict__ kmA,
+                                                 const uint32_t* __restrict__ kmB, int tm, int tn,
+                                                 int kbeg, int kend, unsigned ptid) {
+  constexpr int BM = kP144, BN = kP144, BK = 16, NT = 64 * kPairProducers;
+  using GA = LdsGather<AF, NT>;
+  using GB = LdsGather<BF, NT>;
   auto plane = [&](int buf, int pl) { return smem + (buf * kPPlanes + pl) * kPPlane; };
-  const unsigned tid = threadIdx.x;
   const int64_t i0 = int64_t(tm) * BM, j0 = int64_t(tn) * BN;
   const int K = kend;
   const int nst = (K - kbeg + BK - 1) / BK;
-  uint32_t* __restrict__ kmA = kmap;
-  uint32_t* __restrict__ kmB = kmap + kKMapMax;
-  for (int i = int(tid); i < nst * BK; i += 512) {
-    const int k = kbeg + i < K ? kbeg + i : K - 1;
-    kmA[i] = uint32_t(int64_t(p.csA[k]) - p.min_csA) * 8u;
-    kmB[i] = uint32_t(int64_t(p.rsB[k]) - p.min_rsB) * 8u;
-  }
+  const int nfull = (K - kbeg) / BK;   // stages 0 .. nfull - 1 lie wholly inside K
   const char* __restrict__ Ab = static_cast<const char*>(p.A) + 8 * (p.min_rsA + p.min_csA);
   const char* __restrict__ Bb = static_cast<const char*>(p.B) + 8 * (p.min_rsB + p.min_csB);
-  LdsGather<AF> ga;
-  LdsGather<BF> gb;
-  ga.init(tid, i0, p.M, p.rsA, p.min_rsA);
-  gb.init(tid, j0, p.N, p.csB, p.min_csB);
+  GA ga;
+  GB gb;
+  ga.init(ptid, i0, p.M, p.rsA, p.min_rsA);
+  gb.init(ptid, j0, p.N, p.csB, p.min_csB);
   const bool inside = i0 + BM <= p.M && j0 + BN <= p.N;
   const uint32_t mre = (p.conj_a != 0) == (p.conj_b != 0) ? 0x80000000u : 0u;
 
-  // k offsets of one stage (registers), read from LDS one stage ahead.
-  using KA = std::conditional_t<AF, u32x4[LdsGather<AF>::E], uint32_t>;
-  using KB = std::conditional_t<BF, u32x4[LdsGather<BF>::E], uint32_t>;
-  KA ka;
-  KB kb;
-  auto kread = [&](const uint32_t* km, auto& kx, const auto& gt, int t) {
+  using VA = std::conditional_t<AF, float2[GA::E][4], float2[GA::E]>;
+  using VB = std::conditional_t<BF, float2[GB::E][4], float2[GB::E]>;
+  // Requests stage t (clamped to the last stage: the requests past the end
+  // read valid, unused data).
+  auto fetch = [&](const char* base, const uint32_t* km, const auto& gt, auto& v, int t) {
     using G = std::remove_cv_t<std::remove_reference_t<decltype(gt)>>;
     const int tt = t < nst ? t : nst - 1;
-    if constexpr (std::is_same_v<std::remove_reference_t<decltype(kx)>, uint32_t>) {
-      kx = km[tt * BK + G::kc(tid, 0)];
-    } else {
+    if constexpr (std::is_same_v<std::remove_cv_t<std::remove_reference_t<decltype(v[0])>>,
+                                 float2>) {
+      const uint32_t k1 = km[tt * BK + G::kc(ptid, 0)];
 #pragma unroll
       for (int u = 0; u < G::E; ++u)
-        kx[u] = *reinterpret_cast<const u32x4*>(km + tt * BK + G::kc(tid, u));
-    }
-  };
-  using VA = std::conditional_t<AF, float2[LdsGather<AF>::E][4], float2[LdsGather<AF>::E]>;
-  using VB = std::conditional_t<BF, float2[LdsGather<BF>::E][4], float2[LdsGather<BF>::E]>;
-  VA va;
-  VB vb;
-  auto request = [&](const char* base, const auto& gt, const auto& kx, auto& v) {
-    using G = std::remove_cv_t<std::remove_reference_t<decltype(gt)>>;
+        v[u] = *reinterpret_cast<const float2*>(base + uint64_t(gt.xb[u] + k1));
+    } else {
 #pragma unroll
-    for (int u = 0; u < G::E; ++u) {
-      if constexpr (std::is_same_v<std::remove_cv_t<std::remove_reference_t<decltype(kx)>>,
-                                   uint32_t>) {
-        v[u] = *reinterpret_cast<const float2*>(base + uint64_t(gt.xb[u] + kx));
-      } else {
+      for (int u = 0; u < G::E; ++u) {
+        const u32x4 k4 = *reinterpret_cast<const u32x4*>(km + tt * BK + G::kc(ptid, u));
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          v[u][j] = *reinterpret_cast<const float2*>(base + uint64_t(gt.xb[u] + kx[u][j]));
+          v[u][j] = *reinterpret_cast<const float2*>(base + uint64_t(gt.xb[u] + k4[j]));
       }
     }
   };
+  auto request = [&](int t, VA& va, VB& vb) {
+    fetch(Ab, kmA, ga, va, t);
+    fetch(Bb, kmB, gb, vb, t);
+  };
   // Stage t's values into LDS buffer buf; EDGE zeroes entries outside the
   // problem (rows of edge tiles, k past the end).
-  auto put = [&](const auto& gt, bool is_b, auto edge_tag, int buf, int t, auto& v) {
+  auto put = [&](const auto& gt, bool is_b, auto edge_tag, int buf, int t, const auto& v) {
     using G = std::remove_cv_t<std::remove_reference_t<decltype(gt)>>;
     constexpr bool EDGE = decltype(edge_tag)::value;
-    const int kst = kbeg + t * BK;   // first k of the stage
+    const int kst = kbeg + t * BK;
     uint32_t* ph = plane(buf, is_b ? 2 : 0);
     uint32_t* pl = plane(buf, is_b ? 3 : 1);
     const uint32_t m = is_b ? mre : 0u;
 #pragma unroll
     for (int u = 0; u < G::E; ++u) {
-      const unsigned e = G::elem(tid, u);
-      const int x = G::xc(e), k0 = G::kc(tid, u);
+      const unsigned e = G::elem(ptid, u);
+      const int x = G::xc(e), k0 = G::kc(ptid, u);
       const bool rok = (gt.xok >> u) & 1u;
       if constexpr (std::is_same_v<std::remove_cv_t<std::remove_reference_t<decltype(v[0])>>,
                                    float2>) {
@@ -840,7 +862,7 @@ __device__ __forceinline__ void bf16x3_pair_tile_lds(const KArgs<int32_t>& p,
         if (EDGE && !(rok && kst + k0 < K)) w = float2{0.f, 0.f};
         uint32_t h, l;
         split_pair(w.x, w.y, h, l);
-        if (G::valid(tid, u)) {
+        if (G::valid(ptid, u)) {
           const int a = pword(x, k0);
           ph[a] = h ^ m;
           pl[a] = l ^ m;
@@ -856,7 +878,7 @@ __device__ __forceinline__ void bf16x3_pair_tile_lds(const KArgs<int32_t>& p,
           hi[j] = h ^ m;
           lo[j] = l ^ m;
         }
-        if (G::valid(tid, u)) {
+        if (G::valid(ptid, u)) {
           const int a = pword(x, k0);
           *reinterpret_cast<u32x4*>(ph + a) = hi;
           *reinterpret_cast<u32x4*>(pl + a) = lo;
@@ -864,119 +886,189 @@ __device__ __forceinline__ void bf16x3_pair_tile_lds(const KArgs<int32_t>& p,
       }
     }
   };
-  auto swrite = [&](auto edge_tag, int buf, int t) {
-    put(ga, false, edge_tag, buf, t, va);
-    put(gb, true, edge_tag, buf, t, vb);
+  auto swrite = [&](int buf, int t, const VA& va, const VB& vb) {
+    if (inside && t < nfull) {
+      put(ga, false, std::false_type{}, buf, t, va);
+      put(gb, true, std::false_type{}, buf, t, vb);
+    } else {
+      put(ga, false, std::true_type{}, buf, t, va);
+      put(gb, true, std::true_type{}, buf, t, vb);
+    }
   };
 
-  PairUnits<int32_t, NR, NC> mm;
-  mm.init(p, r0, c0);
-
-  __syncthreads();   // the k maps
-  kread(kmA, ka, ga, 0);
-  kread(kmB, kb, gb, 0);
-  request(Ab, ga, ka, va);
-  request(Bb, gb, kb, vb);
-  kread(kmA, ka, ga, 1);
-  kread(kmB, kb, gb, 1);
-  swrite(std::true_type{}, 0, 0);
-  __syncthreads();
-  // Iteration s: request stage s + 1 (offsets read last iteration), read
-  // the offsets of stage s + 2, multiply stage s, stage s + 1.
-  // With both operands gathered along k, a scheduling barrier keeps the
-  // requests at the top of the stage: the compiler otherwise sinks them
-  // into the products and their latency shows when the stage is staged
-  // (48^4 bdac,dbef: 360 -> 310 us; the same barrier costs the layouts with
-  // a quad gather 4-10 %).
-  auto body = [&](auto edge_tag, int s) {
-    request(Ab, ga, ka, va);
-    request(Bb, gb, kb, vb);
-    kread(kmA, ka, ga, s + 2);
-    kread(kmB, kb, gb, s + 2);
-    if constexpr (!AF && !BF) __builtin_amdgcn_sched_barrier(0);
-    mm.compute(plane(s & 1, 0));
-    swrite(edge_tag, (s + 1) & 1, s + 1);
+  // Three register sets where they fit (B gathered along k): the loads of
+  // the next two stages are in flight while one stage is staged; else two.
+  VA va0, va1, va2;
+  VB vb0, vb1, vb2;
+  if constexpr (BF) {
+    request(0, va0, vb0);
+    request(1, va1, vb1);
+    swrite(0, 0, va0, vb0);
     __syncthreads();
-  };
-  const int nfull = (K - kbeg) / BK;   // stages 0 .. nfull - 1 lie wholly inside K
-  int s = 0;
-  if (inside)
-    for (; s + 1 < nst && s + 1 < nfull; ++s) body(std::false_type{}, s);
-  for (; s + 1 < nst; ++s) body(std::true_type{}, s);
-  mm.compute(plane((nst - 1) & 1, 0));
-  mm.store(p, tm, tn);
+    // Iteration s stages stage s + 1 and requests stage s + 2 (one barrier
+    // each, matching the MFMA waves' loop), unrolled by two so the register
+    // sets alternate statically.
+    for (int s = 0; s + 1 < nst; s += 2) {
+      request(s + 2, va0, vb0);
+      swrite(1, s + 1, va1, vb1);
+      __syncthreads();
+      if (s + 2 >= nst) break;
+      request(s + 3, va1, vb1);
+      swrite(0, s + 2, va0, vb0);
+      __syncthreads();
+    }
+    return;
+  }
+  request(0, va0, vb0);
+  request(1, va1, vb1);
+  request(2, va2, vb2);
+  swrite(0, 0, va0, vb0);
+  __syncthreads();
+  // Iteration s stages stage s + 1 and requests stage s + 3 (one barrier
+  // each), unrolled by three so the register sets rotate statically.
+  for (int s = 0; s + 1 < nst; s += 3) {
+    request(s + 3, va0, vb0);
+    swrite((s + 1) & 1, s + 1, va1, vb1);
+    __syncthreads();
+    if (s + 2 >= nst) break;
+    request(s + 4, va1, vb1);
+    swrite((s + 2) & 1, s + 2, va2, vb2);
+    __syncthreads();
+    if (s + 3 >= nst) break;
+    request(s + 5, va2, vb2);
+    swrite((s + 3) & 1, s + 3, va0, vb0);
+    __syncthreads();
+  }
 }
 
-template <typename Idx, bool AF, bool BF, bool LDSMAP>
-__global__ void __launch_bounds__(512) k_contract_bf16x3_pair(KArgs<Idx> p) {
+template <int NR, int NC>
+__device__ __forceinline__ void pair_ws_consumer(const KArgs<int32_t>& p,
+                                                 const uint32_t* __restrict__ smem, int tm,
+                                                 int tn, int kbeg, int kend, int r0, int c0) {
+  PairUnits<int32_t, NR, NC> mm;
+  mm.init(p, r0, c0);
+  const int nst = (kend - kbeg + 15) / 16;
+  __syncthreads();
+  for (int s = 0; s + 1 < nst; ++s) {
+    mm.compute(smem + (s & 1) * kPPlanes * kPPlane);
+    __syncthreads();
+  }
+  mm.compute(smem + ((nst - 1) & 1) * kPPlanes * kPPlane);
+  mm.template store<true>(p, tm, tn);
+}
+
+template <bool AF, bool BF>
+__global__ void __launch_bounds__(kPairWsThreads) k_contract_bf16x3_pair_ws(KArgs<int32_t> p) {
   __shared__ __attribute__((aligned(16))) uint32_t smem[2 * kPPlanes * kPPlane];
-  __shared__ __attribute__((aligned(16))) uint32_t kmap[LDSMAP ? 2 * kKMapMax : 4];
+  __shared__ __attribute__((aligned(16))) uint32_t kmap[2 * kKMapMax];
   int tm, tn, kbeg, kend;
   tile_and_krange(p, tm, tn, kbeg, kend);
+  // The k maps of this workgroup's range, as byte offsets from the rebased
+  // bases (every wave helps; the loop's first barrier orders them).
+  {
+    const int K = kend;
+    const int n = (K - kbeg + 15) / 16 * 16;
+    for (int i = int(threadIdx.x); i < n; i += kPairWsThreads) {
+      const int k = kbeg + i < K ? kbeg + i : K - 1;
+      kmap[i] = uint32_t(int64_t(p.csA[k]) - p.min_csA) * 8u;
+      kmap[kKMapMax + i] = uint32_t(int64_t(p.rsB[k]) - p.min_rsB) * 8u;
+    }
+  }
+  __syncthreads();
   const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-  const int r0 = kRect144[wid & 7].r0, c0 = kRect144[wid & 7].c0;
-  auto run = [&](auto rect) {
-    constexpr int NR = decltype(rect)::NR, NC = decltype(rect)::NC;
-    if constexpr (LDSMAP)
-      bf16x3_pair_tile_lds<AF, BF, NR, NC>(p, smem, kmap, tm, tn, kbeg, kend, r0, c0);
-    else
-      bf16x3_pair_tile<Idx, AF, BF, NR, NC>(p, smem, tm, tn, kbeg, kend, r0, c0);
-  };
+  if (wid >= kPairConsumers) {
+    pair_ws_producer<AF, BF>(p, smem, kmap, kmap + kKMapMax, tm, tn, kbeg, kend,
+                             unsigned(threadIdx.x) - 64u * kPairConsumers);
+    return;
+  }
+  const int r0 = kRect144[wid].r0, c0 = kRect144[wid].c0;
   switch (wid) {
     case 2:
     case 3:
-      run(Rect<4, 3>{});
+      pair_ws_consumer<4, 3>(p, smem, tm, tn, kbeg, kend, r0, c0);
       break;
     case 6:
-      run(Rect<9, 1>{});
+      pair_ws_consumer<9, 1>(p, smem, tm, tn, kbeg, kend, r0, c0);
       break;
     case 7:
-      run(Rect<4, 2>{});
+      pair_ws_consumer<4, 2>(p, smem, tm, tn, kbeg, kend, r0, c0);
       break;
     default:
-      run(Rect<5, 2>{});
+      pair_ws_consumer<5, 2>(p, smem, tm, tn, kbeg, kend, r0, c0);
       break;
   }
 }
 
-template <typename Idx, bool LDSMAP>
-hipError_t launch_pair144_as(const GemmProblem& g, hipStream_t s) {
-  const KArgs<Idx> a = make_args<Idx>(g, kP144, kP144, 16);
-  const int64_t per = int64_t(a.tiles_m) * a.tiles_n * a.ksplit;
-  const dim3 block(512);
-  const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
-  return launch_batches(a, per, 512, [&](const KArgs<Idx>& q, unsigned grid) {
-    if (af && bf)
-      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, true, true, LDSMAP>), grid, block, 0, s, q);
-    else if (af)
-      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, true, false, LDSMAP>), grid, block, 0, s, q);
-    else if (bf)
-      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, false, true, LDSMAP>), grid, block, 0, s, q);
-    else
-      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, false, false, LDSMAP>), grid, block, 0, s, q);
-  });
+template <typename Idx, bool AF, bool BF>
+__global__ void __launch_bounds__(512) k_contract_bf16x3_pair(KArgs<Idx> p) {
+  __shared__ __attribute__((aligned(16))) uint32_t smem[2 * kPPlanes * kPPlane];
+  int tm, tn, kbeg, kend;
+  tile_and_krange(p, tm, tn, kbeg, kend);
+  const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  const int r0 = kRect144[wid & 7].r0, c0 = kRect144[wid & 7].c0;
+  switch (wid) {
+    case 2:
+    case 3:
+      bf16x3_pair_tile<Idx, AF, BF, 4, 3>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      break;
+    case 6:
+      bf16x3_pair_tile<Idx, AF, BF, 9, 1>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      break;
+    case 7:
+      bf16x3_pair_tile<Idx, AF, BF, 4, 2>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      break;
+    default:
+      bf16x3_pair_tile<Idx, AF, BF, 5, 2>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      break;
+  }
 }
 
-// 32-bit offsets whose operands span less than 2^32 bytes take the LDS-map
-// kernel, in sequential launches of at most kKMapMax contracted indices
-// (each adds its partial products into C; launches on one stream run in
-// order).
+// 32-bit plans whose operands span less than 2^32 bytes take the LDS-map
+// kernel with staging waves, in sequential launches of at most kKMapMax
+// contracted indices (each adds its partial products into C; launches on
+// one stream run in order); others the global-map kernel.
 template <typename Idx>
 hipError_t launch_pair144(const GemmProblem& g, hipStream_t s) {
+  const bool af = g.a_mfast != 0, bf = g.b_kfast == 0;
   if constexpr (std::is_same_v<Idx, int32_t>) {
     if (g.span32) {
+      const dim3 block(kPairWsThreads);
       for (int64_t k0 = 0; k0 < g.K; k0 += kKMapMax) {
         GemmProblem h = g;
         h.K = g.K - k0 < kKMapMax ? g.K - k0 : kKMapMax;
         h.csA = static_cast<const int32_t*>(g.csA) + k0;
         h.rsB = static_cast<const int32_t*>(g.rsB) + k0;
-        const hipError_t e = launch_pair144_as<int32_t, true>(h, s);
+        const KArgs<int32_t> a = make_args<int32_t>(h, kP144, kP144, 16);
+        const int64_t per = int64_t(a.tiles_m) * a.tiles_n * a.ksplit;
+        const hipError_t e = launch_batches(a, per, kPairWsThreads,
+                                            [&](const KArgs<int32_t>& q, unsigned grid) {
+          if (af && bf)
+            hipLaunchKernelGGL((k_contract_bf16x3_pair_ws<true, true>), grid, block, 0, s, q);
+          else if (af)
+            hipLaunchKernelGGL((k_contract_bf16x3_pair_ws<true, false>), grid, block, 0, s, q);
+          else if (bf)
+            hipLaunchKernelGGL((k_contract_bf16x3_pair_ws<false, true>), grid, block, 0, s, q);
+          else
+            hipLaunchKernelGGL((k_contract_bf16x3_pair_ws<false, false>), grid, block, 0, s, q);
+        });
         if (e != hipSuccess) return e;
       }
       return hipSuccess;
     }
   }
-  return launch_pair144_as<Idx, false>(g, s);
+  const KArgs<Idx> a = make_args<Idx>(g, kP144, kP144, 16);
+  const int64_t per = int64_t(a.tiles_m) * a.tiles_n * a.ksplit;
+  const dim3 block(512);
+  return launch_batches(a, per, 512, [&](const KArgs<Idx>& q, unsigned grid) {
+    if (af && bf)
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, true, true>), grid, block, 0, s, q);
+    else if (af)
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, true, false>), grid, block, 0, s, q);
+    else if (bf)
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, false, true>), grid, block, 0, s, q);
+    else
+      hipLaunchKernelGGL((k_contract_bf16x3_pair<Idx, false, false>), grid, block, 0, s, q);
+  });
 }
 
 template <typename Idx, int BM, int BN, int BK, int WGM, int WGN>

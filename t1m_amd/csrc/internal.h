@@ -72,10 +72,10 @@ struct GemmProblem {
   const int64_t* boffA = nullptr;
   const int64_t* boffB = nullptr;
   const int64_t* boffC = nullptr;
-  // Smallest entry of each operand map (element offsets; <= 0), and whether
-  // A's and B's byte spans from those minima stay below 2^32 (kernels may
-  // then address them by 32-bit byte offsets from a rebased base).
-  int64_t min_rsA = 0, min_csA = 0, min_rsB = 0, min_csB = 0;
+  // Smallest entry of each map (element offsets; <= 0), and whether the
+  // byte spans of A, B and C from those minima stay below 2^32 (kernels may
+  // then address them by 32-bit byte offsets from rebased bases).
+  int64_t min_rsA = 0, min_csA = 0, min_rsB = 0, min_csB = 0, min_rsC = 0, min_csC = 0;
   int32_t span32 = 0;
 };
 

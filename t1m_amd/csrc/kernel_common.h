@@ -42,8 +42,8 @@ struct KArgs {
   const int64_t* boffA;   // byte offsets of the batch problems (nbatch > 1)
   const int64_t* boffB;
   const int64_t* boffC;
-  int64_t min_rsA, min_csA, min_rsB, min_csB;   // smallest map entries (<= 0)
-  int32_t span32;   // A and B span < 2^32 bytes from their rebased bases
+  int64_t min_rsA, min_csA, min_rsB, min_csB, min_rsC, min_csC;   // smallest map entries
+  int32_t span32;   // A, B and C span < 2^32 bytes from their rebased bases
 };
 
 // Moves the operand pointers of q to batch problem z (of this launch).
@@ -402,6 +402,7 @@ KArgs<Idx> make_args(const GemmProblem& g, int bm, int bn, int bk = 16) {
   a.boffC = g.L > 1 ? g.boffC : nullptr;
   a.min_rsA = g.min_rsA; a.min_csA = g.min_csA;
   a.min_rsB = g.min_rsB; a.min_csB = g.min_csB;
+  a.min_rsC = g.min_rsC; a.min_csC = g.min_csC;
   a.span32 = g.span32;
   const int64_t tiles = int64_t(a.tiles_m) * a.tiles_n * (g.L > 0 ? g.L : 1);
   const int cus = g.cus > 0 ? g.cus : 256;

@@ -57,7 +57,7 @@ struct Plan {
   int64_t M = 0, N = 0, K = 0, L = 1;
   int a_mfast = 1, b_kfast = 1, idx32 = 1;
   int a_pairs = 0, b_pairs = 0;                  // two-element gathers allowed (GemmProblem)
-  int64_t min_rsA = 0, min_csA = 0, min_rsB = 0, min_csB = 0;   // smallest map entries
+  int64_t min_rsA = 0, min_csA = 0, min_rsB = 0, min_csB = 0, min_rsC = 0, min_csC = 0;
   int span32 = 0;                                // A, B spans < 2^32 bytes (GemmProblem)
   void* maps = nullptr;                          // owned device memory (null: borrowed)
   // Batch problems (L > 1): byte offsets into A, B, C, device [3][L] and host.

@@ -643,6 +643,7 @@ void transpose_problem(GemmProblem& g) {
   const int64_t mra = g.min_rsA, mca = g.min_csA;
   g.min_rsA = g.min_csB; g.min_csA = g.min_rsB;
   g.min_rsB = mca; g.min_csB = mra;
+  std::swap(g.min_rsC, g.min_csC);
 }
 
 // True unless the layout provably gives every index tuple its own element:
@@ -724,11 +725,14 @@ void plan_layout(const Problem& P0, int precision, Plan& pl, Problem& P) {
   pl.min_csA = min_offset(P.k, 0);
   pl.min_rsB = min_offset(P.k, 1);
   pl.min_csB = min_offset(P.n, 1);
+  pl.min_rsC = min_offset(P.m, 2);
+  pl.min_csC = min_offset(P.n, 2);
   // A map entry minus its map's minimum is at most the map's span, so a
-  // row + k byte offset is at most (offX + 1) elements: below 2^32 bytes.
+  // row + column byte offset is at most (offX + 1) elements: below 2^32 bytes.
   const int64_t esz = dtype_size(P.dtype);
   const int64_t lim32 = int64_t(1) << 32;
-  pl.span32 = ((offA + 1) * esz < lim32 && (offB + 1) * esz < lim32) ? 1 : 0;
+  pl.span32 = ((offA + 1) * esz < lim32 && (offB + 1) * esz < lim32 &&
+               (offC + 1) * esz < lim32) ? 1 : 0;
   pl.modes_m = P.m; pl.modes_n = P.n; pl.modes_k = P.k;
   // Two-element gathers (complex<float>): the leading mode (extent > 1) of
   // the operand's gather direction has unit stride and even extent, so
@@ -1132,6 +1136,7 @@ GemmProblem gemm_for(const Plan& pl, const void* A, const void* B, void* C,
   g.idx32 = pl.idx32;
   g.min_rsA = pl.min_rsA; g.min_csA = pl.min_csA;
   g.min_rsB = pl.min_rsB; g.min_csB = pl.min_csB;
+  g.min_rsC = pl.min_rsC; g.min_csC = pl.min_csC;
   g.span32 = pl.idx32 && pl.span32 ? 1 : 0;
   g.dtype = pl.dtype;
   g.split_k = opt.deterministic ? 1 : 0;

@@ -1140,6 +1140,79 @@ def test_offsets_beyond_int32_take_64bit_maps():
         torch.cuda.empty_cache()
 
 
+def _reversed_desc(t, dim):
+    """A descriptor of torch tensor t viewed backwards along `dim` (negative
+    stride, data pointer at the last element of that dimension)."""
+    st = list(t.stride())
+    last = t.data_ptr() + (t.shape[dim] - 1) * st[dim] * t.element_size()
+    st[dim] = -st[dim]
+    return t1m_amd._Desc(t, strides=st, ptr=last)
+
+
+def test_bf16x3_144_staging_paths():
+    """Both staging paths of the 144 x 144 BF16x3 tiles.  Plans with 32-bit
+    offsets whose operands span less than 2^32 bytes run the kernel that
+    copies the contracted-index maps into LDS and addresses A, B and C by
+    32-bit byte offsets from bases rebased by the maps' minima: checked with
+    negative strides on all three operands, interior and edge tiles, K not a
+    multiple of 16, and K past one LDS map (8192 indices: sequential
+    launches).  A 32-bit plan whose operand spans more than 2^32 bytes takes
+    the global-map kernel (both operand roles).  Against complex128 products."""
+    tol = TOL["bf16x3"]
+    g = torch.Generator(device="cuda").manual_seed(77)
+
+    def crandn(*shape):
+        return torch.randn(*shape, dtype=torch.complex64, device="cuda", generator=g)
+
+    lib = t1m_amd.load()
+    for M, N, K in ((300, 290, 37), (160, 150, 8492)):
+        a = crandn(K, M).t()            # col-major M x K ("ik")
+        b = crandn(N, K).t()            # col-major K x N ("kj")
+        c = crandn(N, M).t()            # col-major M x N ("ij")
+        c0 = c.cpu().numpy().astype(np.complex128)
+        # A backwards along i, B backwards along k, C backwards along j
+        da, db, dc = _reversed_desc(a, 0), _reversed_desc(b, 0), _reversed_desc(c, 1)
+        o = t1m_amd._options(precision="bf16x3")
+        with forced("c64_bf16x3_144x144"):
+            rc = lib.t1m_contract(ctypes.byref(da.desc), b"ik", ctypes.byref(db.desc), b"kj",
+                                  ctypes.byref(dc.desc), b"ij", ctypes.byref(o))
+        assert rc == 0, lib.t1m_last_error()
+        An = a.cpu().numpy().astype(np.complex128)[::-1, :]
+        Bn = b.cpu().numpy().astype(np.complex128)[::-1, :]
+        want = c0.copy()
+        want[:, ::-1] += An @ Bn
+        err = rel_err(c.cpu().numpy() - c0, want - c0)
+        print(f"  M={M} N={N} K={K} reversed operands rel={err:.2e}")
+        assert err < tol, (M, N, K, err)
+
+    # offsets below 2^31 elements, byte span above 2^32: 36 * 2^24 + 149
+    step = 1 << 24
+    buf = torch.zeros(36 * step + 160, dtype=torch.complex64, device="cuda")
+    try:
+        big = torch.as_strided(buf, (150, 37), (1, step))
+        big.copy_(crandn(150, 37))
+        bigd = big.contiguous().to(torch.complex128)
+        B = crandn(37, 160)
+        C = torch.zeros(150, 160, dtype=torch.complex64, device="cuda")
+        with forced("c64_bf16x3_144x144"):
+            t1m_amd.contract(big, "ik", B, "kj", C, "ij", precision="bf16x3")
+        ref = bigd @ B.to(torch.complex128)
+        err = rel_err(C.cpu().numpy(), ref.cpu().numpy())
+        print(f"  span > 2^32 bytes as A rel={err:.2e}")
+        assert err < tol, err
+        A = crandn(160, 150)
+        C2 = torch.zeros(160, 37, dtype=torch.complex64, device="cuda")
+        with forced("c64_bf16x3_144x144"):
+            t1m_amd.contract(A, "ik", big, "kj", C2, "ij", precision="bf16x3")
+        ref2 = A.to(torch.complex128) @ bigd
+        err = rel_err(C2.cpu().numpy(), ref2.cpu().numpy())
+        print(f"  span > 2^32 bytes as B rel={err:.2e}")
+        assert err < tol, err
+    finally:
+        del buf
+        torch.cuda.empty_cache()
+
+
 PAIR_CASES = [
     # (labels, extents): both operands pairable (unit stride, even leading
     # extent) in all four gather-direction combinations, ragged tiles and K

@@ -35,6 +35,7 @@ struct BatchPhase {
   int kernel = 0;
   void* d_args = nullptr;
   BatchItem* d_items = nullptr;
+  unsigned* d_tickets = nullptr;   // work-queue counters: one per domain + finished workgroups
   int nitems = 0;    // work-list entries, padding included
   int ntiles = 0;    // tiles (entries that are not padding)
   int ninst = 0;
@@ -61,6 +62,7 @@ struct t1m_batch {
     for (BatchPhase& ph : phases) {
       if (ph.d_args) (void)hipFree(ph.d_args);
       if (ph.d_items) (void)hipFree(ph.d_items);
+      if (ph.d_tickets) (void)hipFree(ph.d_tickets);
     }
     exec = nullptr;
     graph = nullptr;
@@ -238,8 +240,11 @@ int build_phase(t1m_batch& bt, int kern, const std::vector<size_t>& which) {
   BatchPhase& ph = bt.phases.back();   // freed with the batch, also on failure
   ph.kernel = kern;
   ph.ninst = int(which.size());
+  const int ndom = (nx > 1 && slots % nx == 0) ? nx : 1;
   if (hipMalloc(&ph.d_args, args.size()) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&ph.d_items), items.size() * sizeof(BatchItem)) !=
+          hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&ph.d_tickets), sizeof(unsigned) * (ndom + 1)) !=
           hipSuccess) {
     (void)hipGetLastError();
     return set_error(T1M_ERROR_OUT_OF_MEMORY, "hipMalloc for the batch work list failed");
@@ -247,10 +252,12 @@ int build_phase(t1m_batch& bt, int kern, const std::vector<size_t>& which) {
   HIP_TRY(hipMemcpy(ph.d_args, args.data(), args.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ph.d_items, items.data(), items.size() * sizeof(BatchItem),
                     hipMemcpyHostToDevice));
+  // zeroed once here (not in the captured graph); the kernels leave them zero
+  HIP_TRY(hipMemset(ph.d_tickets, 0, sizeof(unsigned) * (ndom + 1)));
   ph.grid = int(std::min<int64_t>(slots, int64_t(items.size())));
   ph.nitems = int(items.size());
   ph.ntiles = int(ntiles);
-  ph.xcds = (nx > 1 && slots % nx == 0) ? nx : 1;
+  ph.xcds = ndom;
   ph.pairs = pairs;
   return T1M_SUCCESS;
 }
@@ -305,7 +312,7 @@ int capture_batch(t1m_batch& bt) {
       const BatchPhase& ph = bt.phases[i];
       e = launch_contract_batched(ph.kernel, g0.idx32 != 0, g0.a_mfast != 0, g0.b_kfast != 0,
                                   ph.pairs, ph.d_args, ph.d_items, ph.nitems, ph.grid,
-                                  bt.capture);
+                                  ph.d_tickets, ph.xcds, bt.capture);
     }
   } else {
     for (size_t i = 0; i < bt.problems.size() && e == hipSuccess; ++i)

@@ -453,23 +453,48 @@ k_contract_1m(KArgs<Idx> p) {
   contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF, false, PAIRS>(p, tm, tn, kbeg, kend);
 }
 
-// Persistent batched form: workgroup g takes items g, g + G, g + 2G, ... of
-// a work list of (instance, tile) sorted by decreasing work, so the grid
-// (a few workgroups per CU) drains it in LPT order in one launch.
+// Persistent batched form over a work queue.  The work list holds ndom
+// interleaved per-domain lists (entry k * ndom + d of domain d, each sorted by
+// decreasing work, padded with instance -1); workgroup b serves domain
+// b mod ndom, the L2 domain (XCD) it runs on, and takes that list's entries
+// in order through the counter tickets[d], so a workgroup whose tiles ran
+// short takes more of them and the grid (a few workgroups per CU) drains
+// the list in LPT order in one launch.  Lane 0 takes the next ticket while
+// the current tile runs.  tickets[ndom] counts finished workgroups; the last
+// one clears every counter, so the next launch (a graph replay) starts from
+// zero without a memset in the graph.  Launches of one list must not
+// overlap.
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
           bool AF, bool BF, bool PAIRS>
 __global__ void __launch_bounds__(64 * WGM * WGN)
 k_contract_1m_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __restrict__ items,
-                      int nitems) {
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const BatchItem w = items[it];
-    if (w.inst < 0) continue;   // padding of an XCD-ordered work list
-    const KArgs<Idx> p = inst[w.inst];
-    // C is prefetched before the main loop on the small FP32 tiles; the
-    // 144 x 144 tiles have no registers for it.
-    contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF, sizeof(T) == 4 && BM < 144, PAIRS>(
-        p, w.tm, w.tn, 0, int(p.K));
-    __syncthreads();   // the next tile rewrites LDS
+                      int nitems, unsigned* __restrict__ tickets, int ndom) {
+  __shared__ int s_ticket;
+  const int d = int(blockIdx.x) % ndom;
+  const int per = nitems / ndom;             // entries of each domain's list
+  if (threadIdx.x == 0) s_ticket = int(atomicAdd(&tickets[d], 1u));
+  __syncthreads();
+  int k = s_ticket;
+  while (k < per) {
+    unsigned next = 0;
+    if (threadIdx.x == 0) next = atomicAdd(&tickets[d], 1u);
+    const BatchItem w = items[int64_t(k) * ndom + d];
+    if (w.inst >= 0) {   // not padding
+      const KArgs<Idx> p = inst[w.inst];
+      // C is prefetched before the main loop on the small FP32 tiles; the
+      // 144 x 144 tiles have no registers for it.
+      contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF, sizeof(T) == 4 && BM < 144,
+                       PAIRS>(p, w.tm, w.tn, 0, int(p.K));
+    }
+    __syncthreads();   // every wave has read s_ticket; the next tile rewrites LDS
+    if (threadIdx.x == 0) s_ticket = int(next);
+    __syncthreads();
+    k = s_ticket;
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&tickets[ndom], 1u) == gridDim.x - 1)
+      for (int i = 0; i <= ndom; ++i) atomicExch(&tickets[i], 0u);
   }
 }
 
@@ -617,7 +642,7 @@ namespace {
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
           bool ALLOW_PAIRS = false>
 struct Batched1M {
-  using K = void (*)(const KArgs<Idx>*, const BatchItem*, int);
+  using K = void (*)(const KArgs<Idx>*, const BatchItem*, int, unsigned*, int);
   static K pick(bool af, bool bf, bool pairs) {
     if constexpr (ALLOW_PAIRS) {
       if (pairs) {
@@ -642,9 +667,10 @@ struct Batched1M {
     return n > 0 ? n : 1;
   }
   static hipError_t launch(bool af, bool bf, bool pairs, const void* inst,
-                           const BatchItem* items, int nitems, int grid, hipStream_t s) {
+                           const BatchItem* items, int nitems, int grid, unsigned* tickets,
+                           int ndom, hipStream_t s) {
     hipLaunchKernelGGL(pick(af, bf, pairs), dim3(grid), dim3(64 * WGM * WGN), 0, s,
-                       static_cast<const KArgs<Idx>*>(inst), items, nitems);
+                       static_cast<const KArgs<Idx>*>(inst), items, nitems, tickets, ndom);
     return hipGetLastError();
   }
 };
@@ -698,19 +724,21 @@ int batch_blocks_per_cu(int32_t kernel, bool idx32, bool af, bool bk, bool pairs
 
 hipError_t launch_contract_batched(int32_t kernel, bool idx32, bool af, bool bk, bool pairs,
                                    const void* inst, const BatchItem* items, int nitems,
-                                   int grid, hipStream_t s) {
+                                   int grid, unsigned* tickets, int ndom, hipStream_t s) {
   const bool bf = !bk;
   if (nitems == 0) return hipSuccess;
   if (kernel == K_C64_BF16X3_144 || kernel == K_C64_BF16X3_64)
     return launch_bf16x3_batched(kernel, idx32, af, bf, inst, items, nitems, grid, s);
   if (is_real_kernel(kernel))
     return launch_real_batched(kernel, idx32, af, bf, inst, items, nitems, grid, s);
+  // every domain needs a workgroup and the list its equal share of entries
+  if (!tickets || ndom < 1 || grid < ndom || nitems % ndom != 0) return hipErrorInvalidValue;
   if (idx32)
     return with_1m_config<int32_t>(kernel, [&](auto c) {
-      return c.launch(af, bf, pairs, inst, items, nitems, grid, s);
+      return c.launch(af, bf, pairs, inst, items, nitems, grid, tickets, ndom, s);
     });
   return with_1m_config<int64_t>(kernel, [&](auto c) {
-    return c.launch(af, bf, pairs, inst, items, nitems, grid, s);
+    return c.launch(af, bf, pairs, inst, items, nitems, grid, tickets, ndom, s);
   });
 }
 

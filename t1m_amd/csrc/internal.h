@@ -126,10 +126,14 @@ void batch_make_args(const GemmProblem& g, int32_t kernel, void* out,
 // Resident workgroups per CU of the batched form of `kernel`.
 // pairs: every instance allows two-element gathers of both operands.
 int batch_blocks_per_cu(int32_t kernel, bool idx32, bool a_mfast, bool b_kfast, bool pairs);
-// Launches the persistent batched kernel over `nitems` work items.
+// Launches the persistent batched kernel over `nitems` work items: `ndom`
+// interleaved per-domain lists (entry k * ndom + d of domain d).  tickets:
+// ndom + 1 zeroed counters (the 1M kernels take the entries of their
+// domain's list through them and leave them zero at exit; the other
+// kernels deal the list statically and ignore them).
 hipError_t launch_contract_batched(int32_t kernel, bool idx32, bool a_mfast,
                                    bool b_kfast, bool pairs, const void* inst,
                                    const BatchItem* items, int nitems, int grid,
-                                   hipStream_t stream);
+                                   unsigned* tickets, int ndom, hipStream_t stream);
 
 }  // namespace t1m_hip

@@ -144,6 +144,8 @@ int t1m_contract_noalloc(const t1m_tensor* a, const char* la, const t1m_tensor* 
  * index maps and an LPT-ordered work list on the device and captures the
  * launch in a HIP graph; launch() replays it.  All data must be device memory,
  * and stays in use by the graph until destroy() (the caller keeps it alive).
+ * Launches of one batch must not overlap (serialise them on one stream): they
+ * add into the same C and share the work list's device counters.
  */
 typedef struct t1m_batch t1m_batch;
 int t1m_batch_create(int64_t count, const t1m_tensor* a, const char* la,

@@ -462,9 +462,29 @@ constexpr int kPW = 16;                // words per LDS row (16 complex k)
 constexpr int kPPlane = kP144 * kPW;   // words per plane
 constexpr int kPPlanes = 4;            // A hi, A lo, B re-hi, B re-lo
 
-// Word offset of word w of LDS row x (chunk swizzle).
+// Word offset of word w of LDS row x (chunk swizzle): rows of 16 words, the
+// 16-byte chunk c of row x at c ^ ((x >> 2) & 3).
 __device__ inline int pword(int x, int w) {
   return x * kPW + ((((w >> 2) ^ (x >> 2)) & 3) << 2) + (w & 3);
+}
+
+// The same plane in units of 16 rows: a unit holds its rows' four 16-byte
+// chunks (words 4c .. 4c + 3) as four blocks of 16 rows, chunk-major.  A
+// fragment read, lane (g, r) taking chunk g of row r0 + r, then puts every
+// 16-lane group on one block in the same lane order, which gfx950 serves
+// without bank conflicts, and so are 16-byte writes of 16 rows of one chunk.
+// Measured with this kernel's accesses in isolation: the row layout above
+// logged 4 conflict cycles per 16-byte read (1.8x the time of these units)
+// and 8 per 16-byte write; its word writes (one k per lane along a row) are
+// conflict-free, the units' are not.  So an operand staged in 16-byte quads
+// (gathered along its rows) uses the units, one staged in words (gathered
+// along k) the rows.
+__device__ inline int pword_units(int x, int w) {
+  return (x >> 4) * 256 + (w >> 2) * 64 + ((x & 15) << 2) + (w & 3);
+}
+template <bool UNITS>
+__device__ inline int plane_word(int x, int w) {
+  return UNITS ? pword_units(x, w) : pword(x, w);
 }
 
 // MFMA side of a 144 x 144 row-pair tile, shared by the staging paths:
@@ -506,8 +526,9 @@ struct PairUnits {
     for (int t = 0; t < 4; ++t) x[t] = __builtin_amdgcn_alignbit(r[t], r[t], 16) ^ mim_r;
     return x;
   }
+  template <bool UNITS>
   __device__ __forceinline__ u32x4 frag(const uint32_t* pl, int row) const {
-    return *reinterpret_cast<const u32x4*>(pl + pword(row, 4 * g));
+    return *reinterpret_cast<const u32x4*>(pl + plane_word<UNITS>(row, 4 * g));
   }
   // Three products per unit and part: lo*hi, hi*lo, hi*hi.
   __device__ __forceinline__ void unit(int r, int c, const u32x4& ah, const u32x4& al,
@@ -520,7 +541,9 @@ struct PairUnits {
     acc[r][c][0] = mfma_bf16(rh, ah, acc[r][c][0]);
     acc[r][c][1] = mfma_bf16(ih, ah, acc[r][c][1]);
   }
-  // One k-chunk from the planes of an LDS stage (A hi, A lo, B re-hi, B re-lo).
+  // One k-chunk from the planes of an LDS stage (A hi, A lo, B re-hi, B re-lo);
+  // UA / UB: A's / B's planes in 16-row units (pword_units), else in rows.
+  template <bool UA = false, bool UB = false>
   __device__ __forceinline__ void compute(const uint32_t* stage) {
     const uint32_t* pl0 = stage;
     const uint32_t* pl1 = stage + kPPlane;
@@ -531,15 +554,15 @@ struct PairUnits {
 #pragma unroll
       for (int x = 0; x < NC; ++x) {
         const int row = (c0 + x) * 16 + l16;
-        h0[x] = frag(pl2, row);
-        h1[x] = frag(pl3, row);
+        h0[x] = frag<UB>(pl2, row);
+        h1[x] = frag<UB>(pl3, row);
         h2[x] = im_form(h0[x]);
         h3[x] = im_form(h1[x]);
       }
 #pragma unroll
       for (int y = 0; y < NR; ++y) {
         const int row = (r0 + y) * 16 + l16;
-        const u32x4 ah = frag(pl0, row), al = frag(pl1, row);
+        const u32x4 ah = frag<UA>(pl0, row), al = frag<UA>(pl1, row);
 #pragma unroll
         for (int x = 0; x < NC; ++x) unit(y, x, ah, al, h0[x], h1[x], h2[x], h3[x]);
       }
@@ -548,13 +571,13 @@ struct PairUnits {
 #pragma unroll
       for (int y = 0; y < NR; ++y) {
         const int row = (r0 + y) * 16 + l16;
-        h0[y] = frag(pl0, row);
-        h1[y] = frag(pl1, row);
+        h0[y] = frag<UA>(pl0, row);
+        h1[y] = frag<UA>(pl1, row);
       }
 #pragma unroll
       for (int x = 0; x < NC; ++x) {
         const int row = (c0 + x) * 16 + l16;
-        const u32x4 rh = frag(pl2, row), rl = frag(pl3, row);
+        const u32x4 rh = frag<UB>(pl2, row), rl = frag<UB>(pl3, row);
         const u32x4 ih = im_form(rh), il = im_form(rl);
 #pragma unroll
         for (int y = 0; y < NR; ++y) unit(y, x, h0[y], h1[y], rh, rl, ih, il);
@@ -878,8 +901,8 @@ __device__ __forceinline__ void pair_ws_producer(const KArgs<int32_t>& p,
           hi[j] = h ^ m;
           lo[j] = l ^ m;
         }
-        if (G::valid(ptid, u)) {
-          const int a = pword(x, k0);
+        if (G::valid(ptid, u)) {   // quads: the 16-row unit layout
+          const int a = pword_units(x, k0);
           *reinterpret_cast<u32x4*>(ph + a) = hi;
           *reinterpret_cast<u32x4*>(pl + a) = lo;
         }
@@ -941,7 +964,7 @@ __device__ __forceinline__ void pair_ws_producer(const KArgs<int32_t>& p,
   }
 }
 
-template <int NR, int NC>
+template <bool AF, bool BF, int NR, int NC>
 __device__ __forceinline__ void pair_ws_consumer(const KArgs<int32_t>& p,
                                                  const uint32_t* __restrict__ smem, int tm,
                                                  int tn, int kbeg, int kend, int r0, int c0) {
@@ -950,10 +973,10 @@ __device__ __forceinline__ void pair_ws_consumer(const KArgs<int32_t>& p,
   const int nst = (kend - kbeg + 15) / 16;
   __syncthreads();
   for (int s = 0; s + 1 < nst; ++s) {
-    mm.compute(smem + (s & 1) * kPPlanes * kPPlane);
+    mm.template compute<AF, BF>(smem + (s & 1) * kPPlanes * kPPlane);
     __syncthreads();
   }
-  mm.compute(smem + ((nst - 1) & 1) * kPPlanes * kPPlane);
+  mm.template compute<AF, BF>(smem + ((nst - 1) & 1) * kPPlanes * kPPlane);
   mm.template store<true>(p, tm, tn);
 }
 
@@ -985,16 +1008,16 @@ __global__ void __launch_bounds__(kPairWsThreads) k_contract_bf16x3_pair_ws(KArg
   switch (wid) {
     case 2:
     case 3:
-      pair_ws_consumer<4, 3>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      pair_ws_consumer<AF, BF, 4, 3>(p, smem, tm, tn, kbeg, kend, r0, c0);
       break;
     case 6:
-      pair_ws_consumer<9, 1>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      pair_ws_consumer<AF, BF, 9, 1>(p, smem, tm, tn, kbeg, kend, r0, c0);
       break;
     case 7:
-      pair_ws_consumer<4, 2>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      pair_ws_consumer<AF, BF, 4, 2>(p, smem, tm, tn, kbeg, kend, r0, c0);
       break;
     default:
-      pair_ws_consumer<5, 2>(p, smem, tm, tn, kbeg, kend, r0, c0);
+      pair_ws_consumer<AF, BF, 5, 2>(p, smem, tm, tn, kbeg, kend, r0, c0);
       break;
   }
 }

@@ -96,17 +96,35 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   using AU = std::conditional_t<AEXP, AExp, C2>;
   constexpr int SA = 256 / int(sizeof(AU));   // units per 256-byte bank sweep
   constexpr int SB = 256 / int(sizeof(C2));
-  constexpr int PADA = AEXP ? ((SA / 2 - BM % SA) % SA + SA) % SA : 4;
-  constexpr int PADB = ((SB / 2 - BN % SB) + SB) % SB;
+  // FP64 (16-byte units): a 16-byte read by 64 lanes is conflict-free on
+  // gfx950 when lane j of each 16-lane group uses the same 16-byte bank slot
+  // ((address / 16) mod 16), i.e. when the groups' rows start at the same
+  // bank slot and share their swizzle.  So A, gathered along its rows (AF),
+  // gets a row pitch of a multiple of 256 bytes (no swizzle: its writes are
+  // whole rows); B, when gathered along its columns, the same pitch with a
+  // swizzle that is one value for the four k rows a read covers,
+  // sw(k) = (k/4) mod 8.  A B gathered along k writes columns, which that
+  // layout serves worse than the pitch-and-swizzle above; F64_BK picks its
+  // layout.
+  constexpr bool F64_BK = false;
+  // (64 x 64 tiles only: the larger pitches would cost the 32 x 32 and thin
+  // tiles a workgroup per CU of LDS)
+  constexpr bool F64L = !AEXP && BM == 64 && BN == 64;
+  constexpr bool F64A = F64L && AF;
+  constexpr bool F64B = F64L && (BF || F64_BK);
+  constexpr int PADA = AEXP ? ((SA / 2 - BM % SA) % SA + SA) % SA : F64A ? 16 : 4;
+  constexpr int PADB = F64B ? 16 : ((SB / 2 - BN % SB) + SB) % SB;
   constexpr int LDA = BM + PADA, LDB = BN + PADB;
   static_assert(!AEXP || (LDA * int(sizeof(AU))) % 256 == 128, "A pitch");
-  static_assert((LDB * int(sizeof(C2))) % 256 == 128, "B pitch");
+  static_assert(!F64A || (LDA * int(sizeof(AU))) % 256 == 0, "A pitch (FP64)");
+  static_assert(F64B ? (LDB * int(sizeof(C2))) % 256 == 0
+                     : (LDB * int(sizeof(C2))) % 256 == 128, "B pitch");
   static_assert(WTM % 8 == 0 && BN % 16 == 0, "tile shape");
   static_assert(NBT - NBW * WGN <= 1, "at most one shared column block");
   static_assert(BK % 4 == 0, "k chunk");
   // A read group spans 8 aligned units, a B group 16.
   auto swa = [](int k) { return AEXP ? (k >> 1) & (SA / 2 - 1 < 7 ? SA / 2 - 1 : 7) : 0; };
-  auto swb = [](int k) { return (k >> 1) & (SB / 2 - 1 < 15 ? SB / 2 - 1 : 15); };
+  auto swb = [](int k) { return (k >> (F64B ? 2 : 1)) & (SB / 2 - 1 < 15 ? SB / 2 - 1 : 15); };
 
   __shared__ AU sA[2][BK][LDA];
   __shared__ C2 sB[2][BK][LDB];

@@ -255,6 +255,29 @@ def test_144_tiles_every_stage_count(layout):
         assert err < TOL["fp32"], (k, err)
 
 
+@pytest.mark.parametrize("layout", [("ik", "kj"), ("ki", "kj"), ("ik", "jk"), ("ki", "jk")])
+def test_bf16x3_144_every_gather_direction(layout):
+    """The 144 x 144 BF16x3 tiles stage an operand gathered along its rows in
+    16-byte quads into a plane of 16-row chunk-major units and one gathered
+    along k in words into a plane of rows; the MFMA waves read each plane in
+    its own layout.  Every pair of gather directions (ik / ki: A along i / k;
+    kj / jk: B along k / j), edge tiles in i and j, K of one to seven stages,
+    ragged or whole, against the oracle."""
+    la, lb = layout
+    rng = np.random.default_rng(13)
+    for k in (1, 16, 37, 100):
+        ext = {"i": 150, "j": 161, "k": k}
+        A = rand(rng, [ext[x] for x in la], np.complex64)
+        B = rand(rng, [ext[x] for x in lb], np.complex64)
+        C0 = rand(rng, [ext[x] for x in "ij"], np.complex64)
+        want = oracle_einsum(A, la, B, lb, C0, "ij")
+        with forced("c64_bf16x3_144x144"):
+            got = run(A, la, B, lb, C0, "ij", precision="bf16x3")
+        err = rel_err(got - C0, want - C0)
+        print(f"  {la},{lb}->ij k={k} rel={err:.2e}")
+        assert err < TOL["bf16x3"], (k, err)
+
+
 REAL_KERNELS = {np.float32: ["f32mfma_128x128", "f32mfma_64x64", "simple"],
                 np.float64: ["f64mfma_64x64", "f64mfma_32x32", "simple"]}
 

@@ -122,9 +122,22 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   static_assert(WTM % 8 == 0 && BN % 16 == 0, "tile shape");
   static_assert(NBT - NBW * WGN <= 1, "at most one shared column block");
   static_assert(BK % 4 == 0, "k chunk");
+  // FP32 64 x 64 tile, 8-k stages, B gathered along k in pairs: lane e
+  // writes (k, k+1) of column e / 4 for k = 2 (e mod 4), so the 32 lanes of
+  // a half-wave write 8 columns of four even (then odd) rows.  A swizzle by
+  // whole 64-byte segments per k pair, and odd k in the other 32-byte half
+  // of its segment, spreads them over the bank sweep: 3.96 conflict cycles
+  // per write instead of 23.5 with the (k/2) mod 16 swizzle (reads of both
+  // layouts: none).  The swizzle then crosses 16-column blocks, so the
+  // fragment reads swizzle the full column within its 32-column pair.
+  constexpr bool BKP = PAIRS && !BF && AEXP && BN == 64 && BK == 8;
   // A read group spans 8 aligned units, a B group 16.
   auto swa = [](int k) { return AEXP ? (k >> 1) & (SA / 2 - 1 < 7 ? SA / 2 - 1 : 7) : 0; };
-  auto swb = [](int k) { return (k >> (F64B ? 2 : 1)) & (SB / 2 - 1 < 15 ? SB / 2 - 1 : 15); };
+  auto swb = [](int k) {
+    if constexpr (BKP) return ((k >> 1) & 3) * 8 + (k & 1) * 4;
+    return (k >> (F64B ? 2 : 1)) & (SB / 2 - 1 < 15 ? SB / 2 - 1 : 15);
+  };
+  static_assert(!BKP || (SB == 32 && LDB >= BN), "B k-pair swizzle: 32-unit bank sweep");
 
   __shared__ AU sA[2][BK][LDA];
   __shared__ C2 sB[2][BK][LDB];
@@ -258,6 +271,14 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   };
   auto load_b = [&](int buf, int kc, BFr& f) {
     const int q = kc * 4 + g;
+    if constexpr (BKP) {
+      static_assert(NBW % 2 == 0 && !XT, "32-column pairs per wave");
+      const int s = swb(q), b0 = bn0 - l16;   // b0: a multiple of 32
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb)
+        f.b[nb] = sB[buf][q][b0 + (nb & ~1) * 16 + (((nb & 1) * 16 + l16) ^ s)];
+      return;
+    }
     const int bsw = (l16 ^ swb(q)) - l16;
 #pragma unroll
     for (int nb = 0; nb < NBW; ++nb) f.b[nb] = sB[buf][q][bn0 + bsw + nb * 16];

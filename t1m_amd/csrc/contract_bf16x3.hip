@@ -103,7 +103,7 @@ struct QuadGather {
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const int64_t xg = x0 + xc(elem(tid, u));
-      xoff[u] = xmap[xg < X ? xg : X - 1];
+      xoff[u] = gmem(xmap)[xg < X ? xg : X - 1];
       xok |= ((xg < X && valid(tid, u)) ? 1u : 0u) << u;
     }
   }
@@ -113,7 +113,7 @@ struct QuadGather {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int kg = p0 + 4 * kq(elem(tid, u)) + t;
-        koff[u][t] = kmap[kg < K ? kg : K - 1];
+        koff[u][t] = gmem(kmap)[kg < K ? kg : K - 1];
       }
   }
   __device__ void load_data(const float2* __restrict__ base) { load_data_to(base, val); }
@@ -121,7 +121,7 @@ struct QuadGather {
 #pragma unroll
     for (int u = 0; u < E; ++u)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) out[u][t] = base[xoff[u] + koff[u][t]];
+      for (int t = 0; t < 4; ++t) out[u][t] = gload(base + (xoff[u] + koff[u][t]));
   }
   // Whether entry (u, t) of a stage starting at p0 lies inside the problem.
   __device__ bool inside(unsigned tid, int p0, int K, int u, int t) const {
@@ -370,7 +370,7 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
     const int blk = r < MB ? r : wn * XB + (r - MB);
     const int64_t i = i0 + am0 + (blk < MB ? blk : MB - 1) * 8;
     rok[r] = i < M && blk < MB;
-    rofs[r] = rok[r] ? p.rsC[i] : Idx(0);
+    rofs[r] = rok[r] ? gmem(p.rsC)[i] : Idx(0);
   }
   Idx cofs[NC][4];
   bool cok[NC][4];
@@ -381,7 +381,7 @@ __device__ __forceinline__ void contract_tile_bf16x3(const KArgs<Idx>& p, int tm
       const int64_t jb = c < NBW ? j0 + wn * NBW * 16 + c * 16 : j0 + NBW * WGN * 16;
       const int64_t j = jb + 4 * g + q;
       cok[c][q] = j < N;
-      cofs[c][q] = cok[c][q] ? p.csC[j] : Idx(0);
+      cofs[c][q] = cok[c][q] ? gmem(p.csC)[j] : Idx(0);
     }
   auto scaled = [&](float v) {
     if (cplx_alpha) {
@@ -631,8 +631,8 @@ struct PairUnits {
             if (rok && cok[c][t]) {
               const float2 v = val(c, t);
               float* d = reinterpret_cast<float*>(at(ro, cofs[c][t]));
-              unsafeAtomicAdd(d, v.x);
-              unsafeAtomicAdd(d + 1, v.y);
+              atomic_add_gmem(d, v.x);
+              atomic_add_gmem(d + 1, v.y);
             }
       } else {
         float2 cv[NC][4];

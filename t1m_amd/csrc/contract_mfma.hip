@@ -372,7 +372,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
       const int blk = r < MB ? r : xlo + ((r - MB) - xlo % XB_ALLOC + XB_ALLOC) % XB_ALLOC;
       const int64_t i = i0 + am0 + (blk < MB ? blk : MB - 1) * 8;
       rok[r] = i < M && (r < MB || blk < xhi);
-      rofs[r] = rok[r] ? p.rsC[i] : Idx(0);
+      rofs[r] = rok[r] ? gmem(p.rsC)[i] : Idx(0);
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -383,7 +383,7 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
         const int64_t jb = c < NBW ? j0 + wn * NBW * 16 + c * 16 : j0 + NBW * WGN * 16;
         const int64_t j = jb + (sizeof(T) == 4 ? 4 * g + q : g + 4 * q);
         cok[c][q] = j < N;
-        cofs[c][q] = cok[c][q] ? p.csC[j] : Idx(0);
+        cofs[c][q] = cok[c][q] ? gmem(p.csC)[j] : Idx(0);
       }
   };
   auto scaled = [&](T v) {

@@ -200,7 +200,7 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
     const int blk = r < MB ? r : xlo + ((r - MB) - xlo % XB_ALLOC + XB_ALLOC) % XB_ALLOC;
     const int64_t i = i0 + am0 + (blk < MB ? blk : MB - 1) * 16;
     rok[r] = i < M && (r < MB || blk < xhi);
-    rofs[r] = rok[r] ? p.rsC[i] : Idx(0);
+    rofs[r] = rok[r] ? gmem(p.rsC)[i] : Idx(0);
   }
   Idx cofs[NC][4];
   bool cok[NC][4];
@@ -212,7 +212,7 @@ __device__ __forceinline__ void contract_tile_real(const KArgs<Idx>& p, int tm, 
       const int64_t jb = c < NBW ? j0 + wn * NBW * 16 + c * 16 : j0 + NBW * WGN * 16;
       const int64_t j = jb + (sizeof(T) == 4 ? 4 * g + q : g + 4 * q);
       cok[c][q] = j < N;
-      cofs[c][q] = cok[c][q] ? p.csC[j] : Idx(0);
+      cofs[c][q] = cok[c][q] ? gmem(p.csC)[j] : Idx(0);
     }
   auto value = [&](int r, int c, int q) -> T {
     if constexpr (XT) {

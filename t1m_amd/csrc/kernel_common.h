@@ -1,6 +1,8 @@
 // Device-side helpers shared by the contraction kernels.
 #pragma once
 
+#include <type_traits>
+
 #include "internal.h"
 
 namespace t1m_hip {
@@ -56,6 +58,42 @@ __device__ inline void batch_shift(KArgs<Idx>& q, int z) {
   }
 }
 
+// Every buffer a kernel here reads or writes through a pointer is global
+// memory, but pointers read from device memory (the persistent kernels'
+// per-instance records) are generic to the compiler, which then emits flat
+// loads, stores and atomics.  A flat operation counts in lgkmcnt as well as
+// vmcnt, so each wait for an LDS operand would also wait for the global loads
+// in flight.  The gathers and the C updates access memory through gmem(),
+// a view of the pointer in the global address space, so the compiler emits
+// global operations whatever the pointer's origin.
+template <typename T>
+using global_t = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ inline global_t<T>* gmem(T* q) {
+  return (global_t<T>*)q;
+}
+// *q read through the global view.  HIP's vector types (float2, double2)
+// are classes whose assignment cannot bind an object of another address
+// space, so their bits are read as a plain vector of words.
+template <typename T>
+__device__ inline T gload(const T* q) {
+  if constexpr (std::is_class_v<T>) {
+    static_assert(sizeof(T) % 4 == 0 && sizeof(T) <= 16, "vector type");
+    typedef unsigned W __attribute__((ext_vector_type(sizeof(T) / 4)));
+    typedef const __attribute__((address_space(1))) W GW;
+    return __builtin_bit_cast(T, *(GW*)q);
+  } else {
+    return *gmem(q);
+  }
+}
+// Unsafe (no-return, denormal-agnostic) atomic add into global memory.
+template <typename T>
+__device__ inline void atomic_add_gmem(T* q, T v) {
+  [[clang::atomic(ignore_denormal_mode)]] {
+    __hip_atomic_fetch_add(gmem(q), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // The value of the neighbouring lane (lanes 2t and 2t+1 swap), by a DPP
 // quad permutation [1, 0, 3, 2]: no LDS round trip.
 __device__ inline float swap_pair(float v) {
@@ -91,7 +129,7 @@ __device__ __forceinline__ void store_tile(bool atomic, Has has, Ok ok, Val val,
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const T v = val(r, c, q);
-            if (ok(r, c, q)) unsafeAtomicAdd(addr(r, c, q), v);
+            if (ok(r, c, q)) atomic_add_gmem(addr(r, c, q), v);
           }
     return;
   }
@@ -104,7 +142,7 @@ __device__ __forceinline__ void store_tile(bool atomic, Has has, Ok ok, Val val,
       for (int c = 0; c < NC; ++c)
         if (has(r, c))
 #pragma unroll
-          for (int q = 0; q < 4; ++q) cv[r - r0][c][q] = *addr(r, c, q);
+          for (int q = 0; q < 4; ++q) cv[r - r0][c][q] = *gmem(addr(r, c, q));
 #pragma unroll
     for (int r = r0; r < r0 + GR && r < NR; ++r)
 #pragma unroll
@@ -113,7 +151,7 @@ __device__ __forceinline__ void store_tile(bool atomic, Has has, Ok ok, Val val,
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const T v = val(r, c, q);
-            if (ok(r, c, q)) *addr(r, c, q) = cv[r - r0][c][q] + v;
+            if (ok(r, c, q)) *gmem(addr(r, c, q)) = cv[r - r0][c][q] + v;
           }
   }
 }
@@ -128,7 +166,7 @@ __device__ __forceinline__ void load_c_tile(Has has, Addr addr, T (&cv)[NR][NC][
     for (int c = 0; c < NC; ++c)
       if (has(r, c))
 #pragma unroll
-        for (int q = 0; q < 4; ++q) cv[r][c][q] = *addr(r, c, q);
+        for (int q = 0; q < 4; ++q) cv[r][c][q] = *gmem(addr(r, c, q));
 }
 
 template <typename T, int NR, int NC, typename Has, typename Ok, typename Val, typename Addr>
@@ -142,7 +180,7 @@ __device__ __forceinline__ void add_store_c_tile(Has has, Ok ok, Val val, Addr a
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const T v = val(r, c, q);
-          if (ok(r, c, q)) *addr(r, c, q) = cv[r][c][q] + v;
+          if (ok(r, c, q)) *gmem(addr(r, c, q)) = cv[r][c][q] + v;
         }
 }
 
@@ -150,9 +188,9 @@ __device__ __forceinline__ void add_store_c_tile(Has has, Ok ok, Val val, Addr a
 template <typename T>
 __device__ inline void accumulate(T* dst, T v, bool atomic) {
   if (atomic)
-    unsafeAtomicAdd(dst, v);
+    atomic_add_gmem(dst, v);
   else
-    *dst += v;
+    *gmem(dst) += v;
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
@@ -215,7 +253,7 @@ struct TileGather {
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const int64_t xg = x0 + xc(elem(tid, u));
-      xoff[u] = xmap[xg < X ? xg : X - 1];
+      xoff[u] = gmem(xmap)[xg < X ? xg : X - 1];
       xok |= ((xg < X && valid(tid, u)) ? 1u : 0u) << u;
     }
   }
@@ -223,13 +261,13 @@ struct TileGather {
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const int kg = p0 + kc(elem(tid, u));
-      koff[u] = kmap[kg < K ? kg : K - 1];
+      koff[u] = gmem(kmap)[kg < K ? kg : K - 1];
     }
   }
   __device__ void load_data(const T2* __restrict__ base) { load_data_to(base, val); }
   __device__ void load_data_to(const T2* __restrict__ base, T2 (&out)[E]) const {
 #pragma unroll
-    for (int u = 0; u < E; ++u) out[u] = base[xoff[u] + koff[u]];
+    for (int u = 0; u < E; ++u) out[u] = gload(base + (xoff[u] + koff[u]));
   }
   // Entry u of a stage starting at p0: zero when out of range, conjugated
   // if asked (sign = -1).
@@ -319,7 +357,7 @@ struct PairGather {
     for (int u = 0; u < E; ++u) {
       const int64_t xg = x0 + xcj(tid, u, 0);
       // an out-of-range pair reads the last pair (XF: X is even) or row
-      xoff[u] = xmap[xg < X ? xg : (XF ? X - 2 : X - 1)];
+      xoff[u] = gmem(xmap)[xg < X ? xg : (XF ? X - 2 : X - 1)];
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int64_t xj = x0 + xcj(tid, u, j);
@@ -331,13 +369,15 @@ struct PairGather {
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const int kg = p0 + kcj(tid, u, 0);
-      koff[u] = kmap[kg < K ? kg : (XF ? K - 1 : K - 2)];
+      koff[u] = gmem(kmap)[kg < K ? kg : (XF ? K - 1 : K - 2)];
     }
   }
   __device__ void load_data(const float2* __restrict__ base) { load_data_to(base, val); }
   __device__ void load_data_to(const float2* __restrict__ base, V (&out)[E]) const {
+    // (the pair is 8-byte aligned: a typed global view keeps that alignment)
+    typedef const __attribute__((address_space(1))) V GV;
 #pragma unroll
-    for (int u = 0; u < E; ++u) out[u] = *reinterpret_cast<const V*>(base + (xoff[u] + koff[u]));
+    for (int u = 0; u < E; ++u) out[u] = *(GV*)(base + (xoff[u] + koff[u]));
   }
   // Element j of slot u of a stage starting at p0: zero when out of range,
   // conjugated if asked (sign = -1).

@@ -74,14 +74,21 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   // MFMA hazard waits); FP64 MFMAs take twice as long and the doubled A tile
   // cost more than the selects (-9 %).
   constexpr bool AEXP = sizeof(T) == 4;
-  // Next stage's LDS writes in the middle of the stage; 64-bit offsets on
-  // the 144 x 144 tiles leave no registers for them.
-  constexpr bool MIDW = sizeof(Idx) == 4 || BM * BN < 144 * 144;
   // Global loads two stages ahead (a second set of data registers) on the
   // 144 x 144 FP32 tiles with 32-bit offsets: +1.2..3.3 % at 2304^3 over
-  // the four gather-direction pairs.  On the 64 x 64 tiles the extra
-  // registers cost a workgroup per CU (-6 %), on FP64 they measured -1..+1 %.
-  constexpr bool DEEP = AEXP && !PREC && BM == 144 && sizeof(Idx) == 4;
+  // the four gather-direction pairs; and on the persistent FP32 batch's
+  // 64 x 64 tiles with two-element gathers, whose kernel is bounded to 3
+  // workgroups per CU (154-167 registers, no spills; one-element gathers
+  // would spill): MPS sweep forward +2.4 %, VJP +1.5 %.  On the
+  // single-launch 64 x 64 tiles the extra registers cost a workgroup per CU
+  // (-6 %; with the same bound -1.8..+2 %), on FP64 they measured -1..+1 %.
+  constexpr bool DEEP = AEXP && (BM == 144 || (PREC && PAIRS)) && sizeof(Idx) == 4;
+  // Next stage's LDS writes in the middle of the stage; 64-bit offsets on
+  // the 144 x 144 tiles leave no registers for them.  The persistent FP32
+  // batch (PREC, with the two-stage prefetch above) writes at the end of the
+  // stage: equal on short K, +1.2 % on the long-K VJP launches of a 512-item
+  // MPS sweep.
+  constexpr bool MIDW = (sizeof(Idx) == 4 || BM * BN < 144 * 144) && !(PREC && DEEP);
   // LDS layouts.  A holds each complex element twice, pre-expanded for the
   // two row parities of 1M: (re, -im) for even lanes, (im, re) for odd ones,
   // so the MFMA operands come straight from the LDS reads.  Both tiles are
@@ -431,6 +438,9 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
     gb.load_data_to(B, vb1);
     ga.load_k(utid, kbeg + 2 * BK, K, p.csA);
     gb.load_k(utid, kbeg + 2 * BK, K, p.rsB);
+    if constexpr (PREC) {
+      if (!atomic) load_c_tile<T, NR, NC>(has, addr, cv);
+    }
     swrite(0, kbeg, ga.val, gb.val);
     __syncthreads();
     auto body = [&](int s, VA& la, VB& lb, const VA& wa, const VB& wb) {
@@ -505,16 +515,19 @@ k_contract_1m(KArgs<Idx> p) {
 // overlap.
 template <typename T, typename Idx, int BM, int BN, int BK, int WGM, int WGN,
           bool AF, bool BF, bool PAIRS>
-__global__ void __launch_bounds__(64 * WGM * WGN)
+// (3 workgroups per CU on the FP32 64 x 64 tiles with 32-bit offsets: the
+// two-stage prefetch must not cost a workgroup)
+__global__ void __launch_bounds__(64 * WGM * WGN,
+                                  (sizeof(T) == 4 && BM < 144 && sizeof(Idx) == 4) ? 3 : 1)
 k_contract_1m_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __restrict__ items,
                       int nitems, unsigned* __restrict__ tickets, int ndom) {
-  __shared__ int s_ticket;
+  __shared__ int s_ticket[2];
   const int d = int(blockIdx.x) % ndom;
   const int per = nitems / ndom;             // entries of each domain's list
-  if (threadIdx.x == 0) s_ticket = int(atomicAdd(&tickets[d], 1u));
+  if (threadIdx.x == 0) s_ticket[0] = int(atomicAdd(&tickets[d], 1u));
   __syncthreads();
-  int k = s_ticket;
-  while (k < per) {
+  int k = s_ticket[0];
+  for (int it = 1; k < per; ++it) {
     unsigned next = 0;
     if (threadIdx.x == 0) next = atomicAdd(&tickets[d], 1u);
     const BatchItem w = items[int64_t(k) * ndom + d];
@@ -525,10 +538,12 @@ k_contract_1m_batched(const KArgs<Idx>* __restrict__ inst, const BatchItem* __re
       contract_tile_1m<T, Idx, BM, BN, BK, WGM, WGN, AF, BF, sizeof(T) == 4 && BM < 144,
                        PAIRS>(p, w.tm, w.tn, 0, int(p.K));
     }
-    __syncthreads();   // every wave has read s_ticket; the next tile rewrites LDS
-    if (threadIdx.x == 0) s_ticket = int(next);
+    // Tickets alternate between two slots, so one barrier both publishes the
+    // next ticket and ends this tile's LDS use: every wave read slot it & 1
+    // (two items ago) before the previous item's closing barrier.
+    if (threadIdx.x == 0) s_ticket[it & 1] = int(next);
     __syncthreads();
-    k = s_ticket;
+    k = s_ticket[it & 1];
   }
   if (threadIdx.x == 0) {
     __threadfence();

@@ -792,6 +792,54 @@ def test_batch_matches_single_calls():
     batch.close()
 
 
+@pytest.mark.parametrize("pairs", ["1", "0"])
+def test_batch_every_stage_count(pairs):
+    """The persistent complex<float> batch over bond dimensions that give one,
+    an odd and an even number of 8-k stages, whole or with a ragged last
+    stage, and ragged tile edges: the forward, grad_A and grad_B launches of
+    an MPS step against the oracle.  With even extents the tiles gather
+    two-element pairs and keep two stages of loads in flight (whose loop
+    peels odd and even stage counts differently); T1M_PAIRS=0 forces the
+    one-element gathers, one stage ahead."""
+    rng = np.random.default_rng(29)
+    la, lb, lc = "asb", "btc", "astc"
+    bonds = [2, 8, 10, 24, 26, 40, 56, 64]     # forward: 1, 1, 2, 3, 4, 5, 7, 8 stages
+    dims = [(int(rng.choice(np.arange(2, 131, 2))), db, int(rng.choice(np.arange(2, 131, 2))))
+            for db in bonds]
+    As = [rand(rng, (dl, 4, db), np.complex64) for dl, db, dr in dims]
+    Bs = [rand(rng, (db, 4, dr), np.complex64) for dl, db, dr in dims]
+    C0s = [rand(rng, (dl, 4, 4, dr), np.complex64) for dl, db, dr in dims]
+    Gs = [rand(rng, (dl, 4, 4, dr), np.complex64) for dl, db, dr in dims]
+    a_dev, b_dev = [to_dev(A) for A in As], [to_dev(B) for B in Bs]
+    c_dev, g_dev = [to_dev(C) for C in C0s], [to_dev(G) for G in Gs]
+    ga_dev = [torch.zeros_like(a) for a in a_dev]
+    gb_dev = [torch.zeros_like(b) for b in b_dev]
+    old = os.environ.get("T1M_PAIRS")
+    os.environ["T1M_PAIRS"] = pairs
+    try:
+        batches = [t1m_amd.Batch(a_dev, la, b_dev, lb, c_dev, lc),
+                   t1m_amd.Batch(g_dev, lc, b_dev, lb, ga_dev, la, conj_b=True),
+                   t1m_amd.Batch(a_dev, la, g_dev, lc, gb_dev, lb, conj_a=True)]
+    finally:
+        if old is None:
+            os.environ.pop("T1M_PAIRS", None)
+        else:
+            os.environ["T1M_PAIRS"] = old
+    for bt in batches:
+        bt.launch()
+    torch.cuda.synchronize()
+    for bt in batches:
+        bt.close()
+    for i, (A, B, C0, G) in enumerate(zip(As, Bs, C0s, Gs)):
+        want_c = oracle_einsum(A, la, B, lb, C0, lc) - C0
+        want_ga = oracle_einsum(G, lc, B, lb, np.zeros_like(A, order="F"), la, conj_b=True)
+        want_gb = oracle_einsum(A, la, G, lc, np.zeros_like(B, order="F"), lb, conj_a=True)
+        print(f"  instance {i} dims {dims[i]}", flush=True)
+        assert rel_err(to_host(c_dev[i]) - C0, want_c) < TOL["fp32"], ("forward", i, dims[i])
+        assert rel_err(to_host(ga_dev[i]), want_ga) < TOL["fp32"], ("grad_A", i, dims[i])
+        assert rel_err(to_host(gb_dev[i]), want_gb) < TOL["fp32"], ("grad_B", i, dims[i])
+
+
 BLOCKED_LAYOUTS = [
     # k: A and B gathered along k, unit strides on different contracted modes
     (("bdac", "dbef", "acef"), dict(a=5, b=8, c=7, d=12, e=6, f=5)),

@@ -423,11 +423,13 @@ int autotune_plan(Plan& pl, const GemmProblem& g0, hipStream_t stream) {
     return set_error(T1M_ERROR_DEVICE, "hipEventCreate for autotuning failed: %s",
                      hipGetErrorString(err));
   }
-  // Each candidate runs as a burst of back-to-back launches (sized to about 2 ms from one
-  // single launch, at most 16): a single launch after an idle GPU pays a start-up cost of
+  // Each candidate runs as a burst of back-to-back launches (sized to about 6 ms from one
+  // single launch, at most 256): a single launch after an idle GPU pays a start-up cost of
   // the order of the 4 % that separates some candidates (44 against 46 us at 1024 x 1024 x
-  // 256 complex<double>).  Two rounds in opposite candidate order, so a clock that is still
-  // settling favours none of them; a candidate's time is its faster round.
+  // 256 complex<double>), and bursts of 16 such launches (0.7 ms) still picked the slower
+  // one on some boxes.  Three rounds, the middle one in the opposite candidate order, so a
+  // clock that is still settling favours none of them; a candidate's time is its fastest
+  // round.
   std::vector<int> todo;
   for (int k : cands) {
     // thin tiles only where C is narrow (elsewhere they cannot win)
@@ -446,16 +448,16 @@ int autotune_plan(Plan& pl, const GemmProblem& g0, hipStream_t stream) {
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
   };
-  for (int round = 0; round < 2 && e == hipSuccess; ++round) {
+  for (int round = 0; round < 3 && e == hipSuccess; ++round) {
     for (size_t i = 0; i < todo.size() && e == hipSuccess; ++i) {
-      const size_t c = round == 0 ? i : todo.size() - 1 - i;
+      const size_t c = round != 1 ? i : todo.size() - 1 - i;
       const int k = todo[c];
       if (round == 0) {
         e = launch_contract(g, k, stream);                 // warm-up
         float one = 0;
         if (e == hipSuccess) time_launches(k, 1, one);
         if (e == hipSuccess)
-          burst[c] = one > 0 ? std::max(1, std::min(16, int(2.0f / one))) : 16;
+          burst[c] = one > 0 ? std::max(1, std::min(256, int(6.0f / one))) : 256;
       }
       float ms = 0;
       if (e == hipSuccess) time_launches(k, burst[c], ms);

@@ -427,7 +427,7 @@ int autotune_plan(Plan& pl, const GemmProblem& g0, hipStream_t stream) {
   // single launch, at most 256): a single launch after an idle GPU pays a start-up cost of
   // the order of the 4 % that separates some candidates (44 against 46 us at 1024 x 1024 x
   // 256 complex<double>), and bursts of 16 such launches (0.7 ms) still picked the slower
-  // one on some boxes.  Three rounds, the middle one in the opposite candidate order, so a
+  // one on some devices.  Three rounds, the middle one in the opposite candidate order, so a
   // clock that is still settling favours none of them; a candidate's time is its fastest
   // round.
   std::vector<int> todo;

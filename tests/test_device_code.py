@@ -7,8 +7,9 @@ code objects (this repository's build, unbundled from t1m_amd/libt1m_hip.so):
   global loads in flight (the persistent batch kernels, whose per-instance
   pointers come from device memory, lost 2-3 % that way);
 * no register spills or scratch in the kernels the headline workloads run
-  (2304^3 complex<float> on the 144 x 144 FP32 and BF16x3 tiles, the FP64
-  16-wave 64 x 64 tiles, the persistent FP32 64 x 64 batch tiles)."""
+  (2304^3 complex<float> on the 144 x 144 FP32 tiles of 8 and 12 waves and
+  the BF16x3 tiles, the FP64 16-wave 64 x 64 tiles, the persistent FP32
+  64 x 64 batch tiles)."""
 import os
 import re
 import subprocess
@@ -26,6 +27,7 @@ TARGET = "hipv4-amdgcn-amd-amdhsa--gfx950"
 # pair variant of each).
 HEADLINE = {
     "FP32 144x144": "_ZN7t1m_hip12_GLOBAL__N_113k_contract_1mIfiLi144ELi144ELi16ELi2ELi4E",
+    "FP32 144x144, 12 waves": "_ZN7t1m_hip12_GLOBAL__N_113k_contract_1mIfiLi144ELi144ELi16ELi3ELi4E",
     "FP64 64x64 16 waves": "_ZN7t1m_hip12_GLOBAL__N_113k_contract_1mIdiLi64ELi64ELi16ELi8ELi2E",
     "FP32 64x64 batch": "_ZN7t1m_hip12_GLOBAL__N_121k_contract_1m_batchedIfiLi64ELi64ELi8ELi2ELi2E",
     "BF16x3 144x144 staging waves": "_ZN7t1m_hip12_GLOBAL__N_125k_contract_bf16x3_pair_ws",

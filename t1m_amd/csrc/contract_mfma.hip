@@ -76,14 +76,16 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   constexpr bool AEXP = sizeof(T) == 4;
   // Global loads two stages ahead (a second set of data registers) on the
   // 8-wave 144 x 144 FP32 tiles with 32-bit offsets: +1.2..3.3 % at 2304^3
-  // over the four gather-direction pairs (the 12-wave grid has no registers
-  // for them); and on the persistent FP32 batch's
+  // over the four gather-direction pairs; on the 12-wave grid (launched only
+  // for operands both gathered along k) +1.7 % at 48^4 bdac,dbef, although
+  // its pair variant spills 2 registers; and on the persistent FP32 batch's
   // 64 x 64 tiles with two-element gathers, whose kernel is bounded to 3
   // workgroups per CU (154-167 registers, no spills; one-element gathers
   // would spill): MPS sweep forward +2.4 %, VJP +1.5 %.  On the
   // single-launch 64 x 64 tiles the extra registers cost a workgroup per CU
   // (-6 %; with the same bound -1.8..+2 %), on FP64 they measured -1..+1 %.
-  constexpr bool DEEP = AEXP && ((BM == 144 && WGM == 2) || (PREC && PAIRS)) && sizeof(Idx) == 4;
+  constexpr bool DEEP = AEXP && ((BM == 144 && (WGM == 2 || (!AF && !BF))) || (PREC && PAIRS)) &&
+                        sizeof(Idx) == 4;
   // Next stage's LDS writes in the middle of the stage; 64-bit offsets on
   // the 144 x 144 tiles leave no registers for them.  The persistent FP32
   // batch (PREC, with the two-stage prefetch above) writes at the end of the
@@ -645,9 +647,10 @@ hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
   switch (kernel) {
     case K_C64_F32_144:
       // Both operands gathered along k: 12 waves (3 x 4, 48 x 36 per wave,
-      // without the two-stage prefetch, 164 registers): 48^4 bdac,dbef->acef
-      // 853 -> 786 us.  The other gather directions keep 8 waves (2 x 4),
-      // which the 3 x 4 grid made 0.5 % slower.
+      // 3 per SIMD): 48^4 bdac,dbef->acef 853 -> 786 us, 774 us with the
+      // two-stage prefetch.  The other gather directions keep 8 waves
+      // (2 x 4), which the 3 x 4 grid without the prefetch made 0.5 % slower
+      // (with it their variants spill 5-12 registers).
       if (sizeof(Idx) == 4 && !g.a_mfast && g.b_kfast)
         return launch_1m<float, Idx, 144, 144, 16, 3, 4, true>(g, s);
       return launch_1m<float, Idx, 144, 144, 16, 2, 4, true>(g, s);

@@ -75,17 +75,16 @@ __device__ __forceinline__ void contract_tile_1m(const KArgs<Idx>& p, int tm, in
   // cost more than the selects (-9 %).
   constexpr bool AEXP = sizeof(T) == 4;
   // Global loads two stages ahead (a second set of data registers) on the
-  // 8-wave 144 x 144 FP32 tiles with 32-bit offsets: +1.2..3.3 % at 2304^3
-  // over the four gather-direction pairs; on the 12-wave grid (launched only
-  // for operands both gathered along k) +1.7 % at 48^4 bdac,dbef, although
-  // its pair variant spills 2 registers; and on the persistent FP32 batch's
-  // 64 x 64 tiles with two-element gathers, whose kernel is bounded to 3
-  // workgroups per CU (154-167 registers, no spills; one-element gathers
-  // would spill): MPS sweep forward +2.4 %, VJP +1.5 %.  On the
-  // single-launch 64 x 64 tiles the extra registers cost a workgroup per CU
-  // (-6 %; with the same bound -1.8..+2 %), on FP64 they measured -1..+1 %.
-  constexpr bool DEEP = AEXP && ((BM == 144 && (WGM == 2 || (!AF && !BF))) || (PREC && PAIRS)) &&
-                        sizeof(Idx) == 4;
+  // 144 x 144 FP32 tiles with 32-bit offsets: on 8 waves +1.2..3.3 % at
+  // 2304^3 over the four gather-direction pairs, on 12 waves +1.4..3.2 %
+  // although the pair variants then spill 2-12 registers; and on the
+  // persistent FP32 batch's 64 x 64 tiles with two-element gathers, whose
+  // kernel is bounded to 3 workgroups per CU (154-167 registers, no spills;
+  // one-element gathers would spill): MPS sweep forward +2.4 %, VJP +1.5 %.
+  // On the single-launch 64 x 64 tiles the extra registers cost a workgroup
+  // per CU (-6 %; with the same bound -1.8..+2 %), on FP64 they measured
+  // -1..+1 %.
+  constexpr bool DEEP = AEXP && (BM == 144 || (PREC && PAIRS)) && sizeof(Idx) == 4;
   // Next stage's LDS writes in the middle of the stage; 64-bit offsets on
   // the 144 x 144 tiles leave no registers for them.  The persistent FP32
   // batch (PREC, with the two-stage prefetch above) writes at the end of the
@@ -646,14 +645,16 @@ template <typename Idx>
 hipError_t dispatch(const GemmProblem& g, int32_t kernel, hipStream_t s) {
   switch (kernel) {
     case K_C64_F32_144:
-      // Both operands gathered along k: 12 waves (3 x 4, 48 x 36 per wave,
-      // 3 per SIMD): 48^4 bdac,dbef->acef 853 -> 786 us, 774 us with the
-      // two-stage prefetch.  The other gather directions keep 8 waves
-      // (2 x 4), which the 3 x 4 grid without the prefetch made 0.5 % slower
-      // (with it their variants spill 5-12 registers).
-      if (sizeof(Idx) == 4 && !g.a_mfast && g.b_kfast)
+      // 32-bit offsets: 12 waves (3 x 4, 48 x 36 per wave, 3 per SIMD) with
+      // the two-stage prefetch, although its pair variants spill 2-12
+      // registers: at 48^4 abcd,dbef / bdac,dbef / abcd,ebdf / bdac,ebdf
+      // 794 / 850 / 793 / 803 us on 8 waves (2 x 4) -> 779 / 775 / 770 / 779
+      // (without the prefetch 12 waves were 795 / 786 / 795 / 803).  64-bit
+      // offsets keep 8 waves.
+      if constexpr (sizeof(Idx) == 4)
         return launch_1m<float, Idx, 144, 144, 16, 3, 4, true>(g, s);
-      return launch_1m<float, Idx, 144, 144, 16, 2, 4, true>(g, s);
+      else
+        return launch_1m<float, Idx, 144, 144, 16, 2, 4, true>(g, s);
     // 8-k stages: 3 workgroups per CU (+5 % and +13 % on two 48^4 / 24^4
     // layouts, -1 % on a third).
     case K_C64_F32_64: return launch_1m<float, Idx, 64, 64, 8, 2, 2, true>(g, s);

@@ -7,9 +7,9 @@ code objects (this repository's build, unbundled from t1m_amd/libt1m_hip.so):
   global loads in flight (the persistent batch kernels, whose per-instance
   pointers come from device memory, lost 2-3 % that way);
 * no register spills or scratch in the kernels the headline workloads run
-  (2304^3 complex<float> on the 144 x 144 FP32 tiles of 8 and 12 waves and
-  the BF16x3 tiles, the FP64 16-wave 64 x 64 tiles, the persistent FP32
-  64 x 64 batch tiles)."""
+  (2304^3 complex<float> on the 144 x 144 FP32 tiles and the BF16x3 tiles,
+  the FP64 16-wave 64 x 64 tiles, the persistent FP32 64 x 64 batch tiles),
+  but for the bounded spills listed below."""
 import os
 import re
 import subprocess
@@ -26,8 +26,8 @@ TARGET = "hipv4-amdgcn-amd-amdhsa--gfx950"
 # Mangled-name prefixes of the headline kernels (every gather-direction and
 # pair variant of each).
 HEADLINE = {
-    "FP32 144x144": "_ZN7t1m_hip12_GLOBAL__N_113k_contract_1mIfiLi144ELi144ELi16ELi2ELi4E",
     "FP32 144x144, 12 waves": "_ZN7t1m_hip12_GLOBAL__N_113k_contract_1mIfiLi144ELi144ELi16ELi3ELi4E",
+    "FP32 144x144, 64-bit offsets": "_ZN7t1m_hip12_GLOBAL__N_113k_contract_1mIflLi144ELi144ELi16ELi2ELi4E",
     "FP64 64x64 16 waves": "_ZN7t1m_hip12_GLOBAL__N_113k_contract_1mIdiLi64ELi64ELi16ELi8ELi2E",
     "FP32 64x64 batch": "_ZN7t1m_hip12_GLOBAL__N_121k_contract_1m_batchedIfiLi64ELi64ELi8ELi2ELi2E",
     "BF16x3 144x144 staging waves": "_ZN7t1m_hip12_GLOBAL__N_125k_contract_bf16x3_pair_ws",
@@ -100,12 +100,17 @@ def kernel_metadata(co):
     return out
 
 
-# The one allowed exception: the 12-wave 144 x 144 tile for operands both
-# gathered along k in pairs keeps the two-stage prefetch although it spills 2
-# registers (12 bytes of scratch), because it is 1.7 % faster with it
-# (48^4 bdac,dbef->acef 787 -> 774 us).
+# The allowed exceptions: the 12-wave 144 x 144 tiles with two-element
+# gathers keep the two-stage prefetch although it makes them spill 2-12
+# registers, because they are faster with it (48^4 abcd,dbef / bdac,dbef /
+# abcd,ebdf / bdac,ebdf: 795 / 786 / 795 / 803 -> 779 / 775 / 770 / 779 us).
+# (mangled-name prefix: most spilled registers, most scratch bytes)
+_W12 = "_ZN7t1m_hip12_GLOBAL__N_113k_contract_1mIfiLi144ELi144ELi16ELi3ELi4E"
 ALLOWED_SPILL = {
-    "_ZN7t1m_hip12_GLOBAL__N_113k_contract_1mIfiLi144ELi144ELi16ELi3ELi4ELb0ELb0ELb1E": (2, 12),
+    _W12 + "Lb1ELb1ELb1E": (5, 24),
+    _W12 + "Lb1ELb0ELb1E": (12, 52),
+    _W12 + "Lb0ELb1ELb1E": (11, 48),
+    _W12 + "Lb0ELb0ELb1E": (2, 12),
 }
 
 
